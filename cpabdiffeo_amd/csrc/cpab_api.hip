@@ -1,0 +1,527 @@
+// C ABI of libcpab_hip.so: context, argument checks, launch configuration and
+// kernel dispatch.  Declarations and their reference counterparts are in
+// include/cpab_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cpab_hip.h"
+#include "cpab_kernels.h"
+
+#pragma clang fp contract(off)
+
+using namespace cpab;
+
+struct cpab_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int max_lds = 65536;
+  int n_cu = 256;
+  bool prof = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;  // pending timed launches
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free;
+  int64_t prof_n = 0;
+  double prof_ms = 0.0;
+  double* scratch = nullptr;  // reduction partials
+  size_t scratch_bytes = 0;
+};
+
+namespace {
+
+int fail(cpab_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  return code;
+}
+
+#define HIP_OK(ctx, call)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return fail(ctx, CPAB_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+bool is_pow2(double x) {
+  if (!(x > 0.0) || !std::isfinite(x)) return false;
+  int e;
+  const double m = std::frexp(x, &e);
+  if (m != 0.5) return false;
+  const double r = 1.0 / x;
+  return std::isfinite(r) && r * x == 1.0;
+}
+
+// Validate the geometry and fill the lookup constants the way the reference
+// kernels compute them inline (cpab/gpu/calc_2Dto2D.cu:55-56,104,124;
+// calc_3Dto3D.cu:57-59).
+int make_geom(cpab_ctx* ctx, int dim, int tess, int nC0, int nC1, int nC2, double ix, double iy,
+              double iz, Geom* g, bool* pow2) {
+  if (dim < 1 || dim > 3) return fail(ctx, CPAB_ERR_UNSUPPORTED, "dim must be 1, 2 or 3 (got %d)", dim);
+  if (tess != TESS_I && tess != TESS_II) return fail(ctx, CPAB_ERR_ARG, "tess must be 1 (I) or 2 (II), got %d", tess);
+  if (tess == TESS_I && dim == 1) return fail(ctx, CPAB_ERR_UNSUPPORTED, "tess I needs dim 2 or 3");
+  const int n[3] = {nC0, nC1, nC2};
+  const double inc[3] = {ix, iy, iz};
+  for (int d = 0; d < dim; d++) {
+    if (n[d] < 1) return fail(ctx, CPAB_ERR_ARG, "nC%d must be >= 1 (got %d)", d, n[d]);
+    if (!(inc[d] > 0.0) || !std::isfinite(inc[d])) return fail(ctx, CPAB_ERR_ARG, "inc %d must be positive and finite", d);
+  }
+  std::memset(g, 0, sizeof *g);
+  g->nC0 = nC0;
+  g->nC1 = nC1;
+  g->nC2 = nC2;
+  g->inc_x = ix;
+  g->inc_y = iy;
+  g->inc_z = iz;
+  bool p2 = true;
+  for (int d = 0; d < dim; d++) p2 = p2 && is_pow2(inc[d]);
+  *pow2 = p2;
+  g->rinc_x = 1.0 / ix;
+  g->rinc_y = 1.0 / iy;
+  g->rinc_z = 1.0 / iz;
+  g->top_x = nC0 * ix - 0.0000000001;
+  g->top_y = nC1 * iy - 0.0000000001;
+  g->top_z = nC2 * ix - 0.0000000001;  // the reference clamps z with inc_x (calc_3Dto3D.cu:59)
+  g->wall_x = nC0 * ix;
+  g->wall_y = nC1 * iy;
+  // The type I clamps must leave a non-empty range, or the box index of a
+  // clamped point would be negative.
+  if (tess == TESS_I && !(g->top_x > 0.0 && g->top_y > 0.0 && (dim < 3 || g->top_z > 0.0)))
+    return fail(ctx, CPAB_ERR_ARG, "cells too small: nC*inc must exceed 1e-10");
+  return CPAB_OK;
+}
+
+int check_nC(cpab_ctx* ctx, int dim, int tess, int nC, int nC0, int nC1, int nC2) {
+  int64_t boxes = nC0;
+  if (dim >= 2) boxes *= nC1;
+  if (dim >= 3) boxes *= nC2;
+  const int64_t want = boxes * (tess == TESS_II ? 1 : (dim == 2 ? 4 : 5));
+  if (nC != want)
+    return fail(ctx, CPAB_ERR_ARG, "nC=%d does not match the tessellation (%lld cells)", nC, (long long)want);
+  return CPAB_OK;
+}
+
+bool overlap(const void* a, size_t na, const void* b, size_t nb) {
+  const char* x = static_cast<const char*>(a);
+  const char* y = static_cast<const char*>(b);
+  return x < y + nb && y < x + na;
+}
+
+// Integration launch shape: 64-lane waves, each owning a contiguous range of
+// points; enough waves for several per SIMD.
+struct Shape {
+  int block;
+  int ptsPerWave;
+  int64_t grid;
+  size_t lds;
+  bool use_lds;
+};
+
+Shape integrate_shape(const cpab_ctx* ctx, int64_t nPts, int nC, int ne, int ntables) {
+  Shape s;
+  const size_t tbytes = (size_t)ntables * nC * ne * sizeof(double);
+  s.use_lds = tbytes <= (size_t)ctx->max_lds;
+  s.lds = s.use_lds ? tbytes : 0;
+  s.block = (s.lds > 32768) ? 1024 : 256;
+  // waves wanted: 8 per SIMD (4 SIMDs per CU); each lane takes ceil of the rest
+  const int64_t want_waves = (int64_t)ctx->n_cu * 4 * 8;
+  int64_t per_lane = (nPts + want_waves * 64 - 1) / (want_waves * 64);
+  if (const char* e = getenv("CPAB_PTS_PER_LANE")) per_lane = atoi(e);
+  if (per_lane < 1) per_lane = 1;
+  s.ptsPerWave = (int)(per_lane * 64);
+  const int64_t waves = (nPts + s.ptsPerWave - 1) / s.ptsPerWave;
+  const int wpb = s.block / 64;
+  s.grid = (waves + wpb - 1) / wpb;
+  return s;
+}
+
+template <typename K>
+void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 65536) (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+int prof_begin(cpab_ctx* ctx, hipEvent_t* a, hipEvent_t* b) {
+  if (!ctx->prof) return CPAB_OK;
+  if (ctx->ev_free.empty()) {
+    hipEvent_t x, y;
+    HIP_OK(ctx, hipEventCreate(&x));
+    HIP_OK(ctx, hipEventCreate(&y));
+    ctx->ev_free.push_back({x, y});
+  }
+  *a = ctx->ev_free.back().first;
+  *b = ctx->ev_free.back().second;
+  ctx->ev_free.pop_back();
+  HIP_OK(ctx, hipEventRecord(*a, ctx->stream));
+  return CPAB_OK;
+}
+
+int prof_end(cpab_ctx* ctx, hipEvent_t a, hipEvent_t b) {
+  if (!ctx->prof) return CPAB_OK;
+  HIP_OK(ctx, hipEventRecord(b, ctx->stream));
+  ctx->ev.push_back({a, b});
+  return CPAB_OK;
+}
+
+template <int DIM, int TESS, bool POW2, bool LDS>
+int launch_integrate_t(cpab_ctx* ctx, int mode, const Shape& s, const double* pos0, double* out,
+                       const double* Trels, const double* As, double h, int nT, int nS, int64_t nPts,
+                       const Geom& g, int nC, int32_t* nfb) {
+  if (mode == 0) {
+    auto k = k_integrate<DIM, TESS, POW2, LDS, 0>;
+    allow_lds(k, s.lds);
+    hipLaunchKernelGGL(k, dim3((unsigned)s.grid), dim3(s.block), s.lds, ctx->stream, pos0, out, Trels, As, h, nT, nS,
+                       nPts, g, nC, s.ptsPerWave, nfb);
+  } else {
+    auto k = k_integrate<DIM, TESS, POW2, LDS, 1>;
+    allow_lds(k, s.lds);
+    hipLaunchKernelGGL(k, dim3((unsigned)s.grid), dim3(s.block), s.lds, ctx->stream, pos0, out, Trels, As, h, nT, nS,
+                       nPts, g, nC, s.ptsPerWave, nfb);
+  }
+  HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+template <int DIM, int TESS>
+int launch_integrate_d(cpab_ctx* ctx, int mode, bool pow2, const Shape& s, const double* pos0, double* out,
+                       const double* Trels, const double* As, double h, int nT, int nS, int64_t nPts,
+                       const Geom& g, int nC, int32_t* nfb) {
+  if (pow2) {
+    if (s.use_lds) return launch_integrate_t<DIM, TESS, true, true>(ctx, mode, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
+    return launch_integrate_t<DIM, TESS, true, false>(ctx, mode, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
+  }
+  if (s.use_lds) return launch_integrate_t<DIM, TESS, false, true>(ctx, mode, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
+  return launch_integrate_t<DIM, TESS, false, false>(ctx, mode, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
+}
+
+int launch_integrate(cpab_ctx* ctx, int mode, int dim, int tess, bool pow2, const Shape& s, const double* pos0,
+                     double* out, const double* Trels, const double* As, double h, int nT, int nS, int64_t nPts,
+                     const Geom& g, int nC, int32_t* nfb) {
+  if (dim == 1) return launch_integrate_d<1, TESS_II>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
+  if (dim == 2) {
+    if (tess == TESS_I) return launch_integrate_d<2, TESS_I>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
+    return launch_integrate_d<2, TESS_II>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
+  }
+  if (tess == TESS_I) return launch_integrate_d<3, TESS_I>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
+  return launch_integrate_d<3, TESS_II>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
+}
+
+// Generic 3-way dispatch for the simple per-point kernels.
+#define DISPATCH_GEOM(dim, tess, pow2, ...)                                          \
+  do {                                                                                 \
+    if ((dim) == 1) {                                                                  \
+      if (pow2) { constexpr int D_ = 1, T_ = TESS_II; constexpr bool P_ = true; __VA_ARGS__; } \
+      else { constexpr int D_ = 1, T_ = TESS_II; constexpr bool P_ = false; __VA_ARGS__; }    \
+    } else if ((dim) == 2) {                                                           \
+      if ((tess) == TESS_I) {                                                          \
+        if (pow2) { constexpr int D_ = 2, T_ = TESS_I; constexpr bool P_ = true; __VA_ARGS__; } \
+        else { constexpr int D_ = 2, T_ = TESS_I; constexpr bool P_ = false; __VA_ARGS__; }   \
+      } else {                                                                         \
+        if (pow2) { constexpr int D_ = 2, T_ = TESS_II; constexpr bool P_ = true; __VA_ARGS__; } \
+        else { constexpr int D_ = 2, T_ = TESS_II; constexpr bool P_ = false; __VA_ARGS__; }  \
+      }                                                                                \
+    } else {                                                                           \
+      if ((tess) == TESS_I) {                                                          \
+        if (pow2) { constexpr int D_ = 3, T_ = TESS_I; constexpr bool P_ = true; __VA_ARGS__; } \
+        else { constexpr int D_ = 3, T_ = TESS_I; constexpr bool P_ = false; __VA_ARGS__; }   \
+      } else {                                                                         \
+        if (pow2) { constexpr int D_ = 3, T_ = TESS_II; constexpr bool P_ = true; __VA_ARGS__; } \
+        else { constexpr int D_ = 3, T_ = TESS_II; constexpr bool P_ = false; __VA_ARGS__; }  \
+      }                                                                                \
+    }                                                                                  \
+  } while (0)
+
+int ensure_scratch(cpab_ctx* ctx, size_t bytes) {
+  if (ctx->scratch_bytes >= bytes) return CPAB_OK;
+  if (ctx->scratch) {
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_OK(ctx, hipFree(ctx->scratch));
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+  }
+  HIP_OK(ctx, hipMalloc(&ctx->scratch, bytes));
+  ctx->scratch_bytes = bytes;
+  return CPAB_OK;
+}
+
+}  // namespace
+
+// ====================================================================== API
+extern "C" {
+
+const char* cpab_version(void) { return "cpab_hip 0.1 gfx950 (fp64 integration, no fp contraction)"; }
+
+int cpab_create(int device, cpab_ctx** out) {
+  if (!out) return CPAB_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return CPAB_ERR_HIP;
+  if (device < 0 || device >= n) return CPAB_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return CPAB_ERR_HIP;
+  cpab_ctx* ctx = new (std::nothrow) cpab_ctx();
+  if (!ctx) return CPAB_ERR_NOMEM;
+  ctx->device = device;
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+    ctx->max_lds = v;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0) ctx->n_cu = v;
+  *out = ctx;
+  return CPAB_OK;
+}
+
+int cpab_destroy(cpab_ctx* ctx) {
+  if (!ctx) return CPAB_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  else (void)hipDeviceSynchronize();
+  for (auto& p : ctx->ev) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
+  for (auto& p : ctx->ev_free) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  delete ctx;
+  return CPAB_OK;
+}
+
+const char* cpab_last_error(const cpab_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cpab_set_stream(cpab_ctx* ctx, void* stream) {
+  if (!ctx) return CPAB_ERR_ARG;
+  ctx->stream = static_cast<hipStream_t>(stream);
+  return CPAB_OK;
+}
+
+int cpab_synchronize(cpab_ctx* ctx) {
+  if (!ctx) return CPAB_ERR_ARG;
+  HIP_OK(ctx, hipSetDevice(ctx->device));
+  HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+  return CPAB_OK;
+}
+
+int cpab_prep_trels(cpab_ctx* ctx, int dim, int nC, const double* As, int sign, double dt, double* sA_out,
+                    double* T_out) {
+  if (!ctx) return CPAB_ERR_ARG;
+  if (dim < 1 || dim > 3) return fail(ctx, CPAB_ERR_UNSUPPORTED, "dim must be 1, 2 or 3");
+  if (nC < 1) return fail(ctx, CPAB_ERR_ARG, "nC must be >= 1");
+  if (sign != 1 && sign != -1) return fail(ctx, CPAB_ERR_ARG, "sign must be +1 or -1");
+  if (!As || !T_out) return fail(ctx, CPAB_ERR_ARG, "null table");
+  const size_t nb = (size_t)nC * dim * (dim + 1) * sizeof(double);
+  if (overlap(As, nb, T_out, nb) || (sA_out && overlap(sA_out, nb, T_out, nb)))
+    return fail(ctx, CPAB_ERR_ALIAS, "Trels_out overlaps an input");
+  const dim3 grid((nC + 255) / 256, 1), block(256);
+  if (dim == 1) hipLaunchKernelGGL(k_prep<1>, grid, block, 0, ctx->stream, As, nC, sign, dt, sA_out, T_out, 12, (int64_t)0);
+  else if (dim == 2) hipLaunchKernelGGL(k_prep<2>, grid, block, 0, ctx->stream, As, nC, sign, dt, sA_out, T_out, 12, (int64_t)0);
+  else hipLaunchKernelGGL(k_prep<3>, grid, block, 0, ctx->stream, As, nC, sign, dt, sA_out, T_out, 12, (int64_t)0);
+  HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+int cpab_expm(cpab_ctx* ctx, int dim, int nC, const double* As, double* Ts, int p) {
+  if (!ctx) return CPAB_ERR_ARG;
+  if (dim != 2 && dim != 3) return fail(ctx, CPAB_ERR_UNSUPPORTED, "expm: dim must be 2 or 3");
+  if (nC < 1 || p < 1) return fail(ctx, CPAB_ERR_ARG, "expm: nC and p must be >= 1");
+  const dim3 grid((nC + 511) / 512), block(512);
+  if (dim == 2) hipLaunchKernelGGL(k_expm<2>, grid, block, 0, ctx->stream, As, Ts, nC, p);
+  else hipLaunchKernelGGL(k_expm<3>, grid, block, 0, ctx->stream, As, Ts, nC, p);
+  HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+static int integrate_common(cpab_ctx* ctx, int mode, int dim, int tess, int nC, const double* pos0, double* out,
+                            const double* Trels, const double* As, double dt, int nT, int nS, int64_t nPts,
+                            int nC0, int nC1, int nC2, double ix, double iy, double iz, int32_t* nfb) {
+  if (!ctx) return CPAB_ERR_ARG;
+  Geom g;
+  bool pow2;
+  int r = make_geom(ctx, dim, tess, nC0, nC1, nC2, ix, iy, iz, &g, &pow2);
+  if (r) return r;
+  if ((r = check_nC(ctx, dim, tess, nC, nC0, nC1, nC2))) return r;
+  if (nPts < 0) return fail(ctx, CPAB_ERR_ARG, "nPts must be >= 0");
+  if (nT < 0 || nS < 0) return fail(ctx, CPAB_ERR_ARG, "nTimeSteps and nStepsODE must be >= 0");
+  if (nPts == 0) return CPAB_OK;
+  if (!pos0 || !out || !Trels || !As) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  const size_t pb = (size_t)nPts * dim * sizeof(double);
+  const size_t ob = mode == 0 ? pb : pb * (size_t)(nT > 0 ? nT : 1);
+  if (overlap(pos0, pb, out, ob)) return fail(ctx, CPAB_ERR_ALIAS, "pts and out cannot point to the same memory");
+  const int ne = dim * (dim + 1);
+  const Shape s = integrate_shape(ctx, nPts, nC, ne, 2);
+  if (s.grid > 0x7fffffffLL) return fail(ctx, CPAB_ERR_ARG, "too many points for one launch");
+  // h = dt/double(nStepsOdeSolver) (calc_2Dto2D.cu:383)
+  const double h = dt / (double)nS;
+  hipEvent_t a = nullptr, b = nullptr;
+  if ((r = prof_begin(ctx, &a, &b))) return r;
+  if ((r = launch_integrate(ctx, mode, dim, tess, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb))) return r;
+  return prof_end(ctx, a, b);
+}
+
+int cpab_calc_T(cpab_ctx* ctx, int dim, int tess, int nC, const double* pos0, double* pos, const double* Trels,
+                const double* As, double dt, int nT, int nS, int64_t nPts, int nC0, int nC1, int nC2, double ix,
+                double iy, double iz, int32_t* n_fallback) {
+  return integrate_common(ctx, 0, dim, tess, nC, pos0, pos, Trels, As, dt, nT, nS, nPts, nC0, nC1, nC2, ix, iy, iz,
+                          n_fallback);
+}
+
+int cpab_calc_trajectory(cpab_ctx* ctx, int dim, int tess, int nC, const double* pos0, double* traj,
+                         const double* Trels, const double* As, double dt, int nT, int nS, int64_t nPts, int nC0,
+                         int nC1, int nC2, double ix, double iy, double iz) {
+  return integrate_common(ctx, 1, dim, tess, nC, pos0, traj, Trels, As, dt, nT, nS, nPts, nC0, nC1, nC2, ix, iy, iz,
+                          nullptr);
+}
+
+int cpab_calc_T_simple(cpab_ctx* ctx, int dim, int tess, int nC, const double* pos0, double* pos, const double* As,
+                       double dt, int nT, int nS, int64_t nPts, int nC0, int nC1, int nC2, double ix, double iy,
+                       double iz) {
+  if (!ctx) return CPAB_ERR_ARG;
+  Geom g;
+  bool pow2;
+  int r = make_geom(ctx, dim, tess, nC0, nC1, nC2, ix, iy, iz, &g, &pow2);
+  if (r) return r;
+  if ((r = check_nC(ctx, dim, tess, nC, nC0, nC1, nC2))) return r;
+  if (nPts < 0 || nT < 0 || nS < 0) return fail(ctx, CPAB_ERR_ARG, "negative size");
+  if (nPts == 0) return CPAB_OK;
+  if (!pos0 || !pos || !As) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  const size_t pb = (size_t)nPts * dim * sizeof(double);
+  if (overlap(pos0, pb, pos, pb)) return fail(ctx, CPAB_ERR_ALIAS, "pts and out cannot point to the same memory");
+  const int64_t steps64 = (int64_t)nS * nT;
+  if (steps64 > 0x7fffffffLL) return fail(ctx, CPAB_ERR_ARG, "nStepsODE*nTimeSteps overflows int");
+  const double h = dt / (double)nS;
+  const size_t tb = (size_t)nC * dim * (dim + 1) * sizeof(double);
+  const bool lds = tb <= (size_t)ctx->max_lds;
+  const int block = 256;
+  const dim3 grid((unsigned)((nPts + block - 1) / block));
+  hipEvent_t a = nullptr, b = nullptr;
+  if ((r = prof_begin(ctx, &a, &b))) return r;
+  DISPATCH_GEOM(dim, tess, pow2, {
+    if (lds) {
+      auto k = k_simple<D_, T_, P_, true>;
+      allow_lds(k, tb);
+      hipLaunchKernelGGL(k, grid, dim3(block), tb, ctx->stream, pos0, pos, As, h, (int)steps64, nPts, g, nC);
+    } else {
+      hipLaunchKernelGGL((k_simple<D_, T_, P_, false>), grid, dim3(block), 0, ctx->stream, pos0, pos, As, h,
+                         (int)steps64, nPts, g, nC);
+    }
+  });
+  HIP_OK(ctx, hipGetLastError());
+  return prof_end(ctx, a, b);
+}
+
+int cpab_calc_v(cpab_ctx* ctx, int dim, int tess, int nC, const double* pts, double* vel, const double* As,
+                int64_t nPts, int nC0, int nC1, int nC2, double ix, double iy, double iz) {
+  if (!ctx) return CPAB_ERR_ARG;
+  Geom g;
+  bool pow2;
+  int r = make_geom(ctx, dim, tess, nC0, nC1, nC2, ix, iy, iz, &g, &pow2);
+  if (r) return r;
+  if ((r = check_nC(ctx, dim, tess, nC, nC0, nC1, nC2))) return r;
+  if (nPts < 0) return fail(ctx, CPAB_ERR_ARG, "nPts must be >= 0");
+  if (nPts == 0) return CPAB_OK;
+  if (!pts || !vel || !As) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  const size_t pb = (size_t)nPts * dim * sizeof(double);
+  if (overlap(pts, pb, vel, pb)) return fail(ctx, CPAB_ERR_ALIAS, "pts and v cannot point to the same memory");
+  const size_t tb = (size_t)nC * dim * (dim + 1) * sizeof(double);
+  const bool lds = tb <= (size_t)ctx->max_lds;
+  const int block = 256;
+  const dim3 grid((unsigned)((nPts + block - 1) / block));
+  DISPATCH_GEOM(dim, tess, pow2, {
+    if (lds) {
+      auto k = k_velocity<D_, T_, P_, true>;
+      allow_lds(k, tb);
+      hipLaunchKernelGGL(k, grid, dim3(block), tb, ctx->stream, pts, vel, As, nPts, g, nC);
+    } else {
+      hipLaunchKernelGGL((k_velocity<D_, T_, P_, false>), grid, dim3(block), 0, ctx->stream, pts, vel, As, nPts, g,
+                         nC);
+    }
+  });
+  HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+int cpab_calc_cell_idx(cpab_ctx* ctx, int dim, int tess, const double* pts, int32_t* cell, int64_t nPts, int nC0,
+                       int nC1, int nC2, double ix, double iy, double iz) {
+  if (!ctx) return CPAB_ERR_ARG;
+  Geom g;
+  bool pow2;
+  int r = make_geom(ctx, dim, tess, nC0, nC1, nC2, ix, iy, iz, &g, &pow2);
+  if (r) return r;
+  if (nPts < 0) return fail(ctx, CPAB_ERR_ARG, "nPts must be >= 0");
+  if (nPts == 0) return CPAB_OK;
+  if (!pts || !cell) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  const int block = 256;
+  const dim3 grid((unsigned)((nPts + block - 1) / block));
+  DISPATCH_GEOM(dim, tess, pow2, {
+    hipLaunchKernelGGL((k_cell_idx<D_, T_, P_>), grid, dim3(block), 0, ctx->stream, pts, cell, nPts, g);
+  });
+  HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+int cpab_remap2d(cpab_ctx* ctx, const double* pts, const float* img, float* out, int H, int W) {
+  if (!ctx) return CPAB_ERR_ARG;
+  if (H < 1 || W < 1) return fail(ctx, CPAB_ERR_ARG, "image must be non-empty");
+  if (!pts || !img || !out) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  const int64_t n = (int64_t)H * W;
+  if (overlap(img, n * 4, out, n * 4)) return fail(ctx, CPAB_ERR_ALIAS, "img and out cannot share memory");
+  hipLaunchKernelGGL(k_remap2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, pts, img, out, H, W);
+  HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+int cpab_remap3d(cpab_ctx* ctx, const double* pts, const float* vol, float* out, int D, int H, int W) {
+  if (!ctx) return CPAB_ERR_ARG;
+  if (D < 1 || H < 1 || W < 1) return fail(ctx, CPAB_ERR_ARG, "volume must be non-empty");
+  if (!pts || !vol || !out) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  const int64_t n = (int64_t)D * H * W;
+  if (overlap(vol, n * 4, out, n * 4)) return fail(ctx, CPAB_ERR_ALIAS, "vol and out cannot share memory");
+  hipLaunchKernelGGL(k_remap3d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, pts, vol, out, D, H,
+                     W);
+  HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+int cpab_ssd_ll(cpab_ctx* ctx, const float* x, const float* y, int64_t n, double sigma, double* ll) {
+  if (!ctx) return CPAB_ERR_ARG;
+  if (n < 1) return fail(ctx, CPAB_ERR_ARG, "n must be >= 1");
+  if (!(sigma > 0.0)) return fail(ctx, CPAB_ERR_ARG, "sigma must be positive");
+  if (!x || !y || !ll) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  int nb = (int)((n + 255) / 256);
+  if (nb > 1024) nb = 1024;
+  int r = ensure_scratch(ctx, (size_t)nb * sizeof(double));
+  if (r) return r;
+  hipLaunchKernelGGL(k_ssd_partial, dim3(nb), dim3(256), 0, ctx->stream, x, y, n, ctx->scratch);
+  hipLaunchKernelGGL(k_reduce_ll, dim3(1), dim3(64), 0, ctx->stream, ctx->scratch, nb, sigma, ll);
+  HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+int cpab_prof_enable(cpab_ctx* ctx, int on) {
+  if (!ctx) return CPAB_ERR_ARG;
+  ctx->prof = on != 0;
+  return CPAB_OK;
+}
+
+int cpab_prof_read(cpab_ctx* ctx, int64_t* n, double* ms) {
+  if (!ctx) return CPAB_ERR_ARG;
+  double total = ctx->prof_ms;
+  int64_t cnt = ctx->prof_n;
+  for (auto& p : ctx->ev) {
+    HIP_OK(ctx, hipEventSynchronize(p.second));
+    float t = 0.f;
+    HIP_OK(ctx, hipEventElapsedTime(&t, p.first, p.second));
+    total += t;
+    cnt++;
+    ctx->ev_free.push_back(p);
+  }
+  ctx->ev.clear();
+  ctx->prof_ms = 0.0;
+  ctx->prof_n = 0;
+  if (n) *n = cnt;
+  if (ms) *ms = total;
+  return CPAB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,468 @@
+// CDNA4 kernels of the CPA integration path.
+//
+//  * k_prep        sign * As, dt scaling and the per-cell matrix exponential
+//                  in one launch (cpab/cpaNd/_CpaCalcs.py:243-313 +
+//                  cpab/gpu/expm/expm_affine_by_series_{2D,3D}.py).
+//  * k_integrate   calc_T / calc_trajectory (cpab/gpu/calc_2Dto2D.cu:348-408,
+//                  499-556).  One lane owns one point at a time; the per-cell
+//                  tables live in LDS.  The reference runs every step as
+//                  "Trel proposal, and on a cell change nStepsODE RK2
+//                  substeps", which in SIMT form idles 63 lanes of a wave
+//                  while one lane does its ten substeps.  Here every loop trip
+//                  of a lane is ONE unit of that schedule (a proposal or a
+//                  substep), each with exactly one cell lookup, so lanes that
+//                  cross a facet drift out of step with their neighbours
+//                  instead of stalling them, and a lane that finishes its
+//                  point takes the next point of its wave's range.  Per
+//                  point, the sequence of IEEE operations is the reference's.
+//  * k_simple      calc_T_simple (calc_2Dto2D.cu:412-445).
+//  * k_velocity    calc_v (calc_2Dto2D.cu:561-602).
+//  * k_cell_idx    calc_cell_idx (calc_2Dto2D.cu:323-344).
+//  * k_remap2d / k_remap3d / k_ssd_partial / k_reduce_ll: remap_fwd and the
+//                  Gaussian SSD likelihood in FP32 with an FP64 final sum.
+#pragma once
+#include "cpab_device.h"
+
+#pragma clang fp contract(off)
+
+namespace cpab {
+
+// ------------------------------------------------------------------ expm
+// expm_affine_by_series_2D.py:28-74
+__device__ inline void expm_2x2(double a, double b, double c, double d, double* E) {
+  const double delta_tmp = (a - d) * (a - d) + 4 * b * c;
+  const double e = exp((a + d) / 2);
+  if (delta_tmp == 0) {
+    E[0] = (1 + (a - d) / 2) * e;
+    E[1] = b * e;
+    E[2] = c * e;
+    E[3] = (1 - (a - d) / 2) * e;
+  } else if (delta_tmp > 0) {
+    const double delta = sqrt(delta_tmp) / 2;
+    const double ch = cosh(delta), s = sinh(delta) / delta;
+    E[0] = (ch + (a - d) / 2 * s) * e;
+    E[1] = b * s * e;
+    E[2] = c * s * e;
+    E[3] = (ch - (a - d) / 2 * s) * e;
+  } else {
+    const double delta = sqrt(-delta_tmp) / 2;
+    const double co = cos(delta), s = sin(delta) / delta;
+    E[0] = (co + (a - d) / 2 * s) * e;
+    E[1] = b * s * e;
+    E[2] = c * s * e;
+    E[3] = (co - (a - d) / 2 * s) * e;
+  }
+}
+
+// expm_affine_by_series_2D.py:85-237 (closed form when the linear part is
+// invertible, else the p-term Taylor series of the 2x3 block)
+__device__ inline void expm_affine2(const double* A, double* T, int p) {
+  const double A00 = A[0], A01 = A[1], A02 = A[2], A10 = A[3], A11 = A[4], A12 = A[5];
+  const double det = A00 * A11 - A01 * A10;
+  if (det != 0) {
+    double E[4];
+    expm_2x2(A00, A01, A10, A11, E);
+    T[0] = E[0];
+    T[1] = E[1];
+    T[3] = E[2];
+    T[4] = E[3];
+    const double i00 = A11 / det, i11 = A00 / det, i01 = -A01 / det, i10 = -A10 / det;
+    const double a = i00 * A02 + i01 * A12;
+    const double b = i10 * A02 + i11 * A12;
+    T[2] = (T[0] - 1) * a + T[1] * b;
+    T[5] = T[3] * a + (T[4] - 1) * b;
+    return;
+  }
+  double S[6] = {1, 0, 0, 0, 1, 0};  // identity, then + A^1 + A^2/2! + ...
+  double An[6] = {A00, A01, A02, A10, A11, A12};
+  for (int k = 0; k < 6; k++) S[k] += An[k];
+  int fact = 1;
+  for (int i = 2; i <= p; i++) {
+    const double B0 = An[0], B1 = An[1], B2 = An[2], B3 = An[3], B4 = An[4], B5 = An[5];
+    fact *= i;
+    An[0] = A00 * B0 + A01 * B3;
+    An[1] = A00 * B1 + A01 * B4;
+    An[2] = A00 * B2 + A01 * B5;
+    An[3] = A10 * B0 + A11 * B3;
+    An[4] = A10 * B1 + A11 * B4;
+    An[5] = A10 * B2 + A11 * B5;
+    for (int k = 0; k < 6; k++) S[k] += An[k] / fact;
+  }
+  for (int k = 0; k < 6; k++) T[k] = S[k];
+}
+
+// expm_affine_by_series_3D.py:27-219 (series on the 3x4 block)
+__device__ inline void expm_affine3(const double* A, double* T, int p) {
+  double S[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+  double An[12];
+  for (int k = 0; k < 12; k++) An[k] = A[k];
+  for (int k = 0; k < 12; k++) S[k] += An[k];
+  int fact = 1;
+  for (int i = 2; i <= p; i++) {
+    double B[12];
+    for (int k = 0; k < 12; k++) B[k] = An[k];
+    fact *= i;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int col = 0; col < 4; col++)
+        An[r * 4 + col] = A[r * 4 + 0] * B[col] + A[r * 4 + 1] * B[4 + col] + A[r * 4 + 2] * B[8 + col];
+    for (int k = 0; k < 12; k++) S[k] += An[k] / fact;
+  }
+  for (int k = 0; k < 12; k++) T[k] = S[k];
+}
+
+// Fused prep of cpab/cpaNd/_CpaCalcs.py:243-313: signedAs = sign*As (the
+// reference multiplies by -1 only for the inverse), As_dt = signedAs*dt,
+// Trels = expm(As_dt); in 1D the closed form of :285-292.  One lane per
+// cell; `inst` strides over independent instances (batched callers).
+template <int DIM>
+__global__ void k_prep(const double* __restrict__ As, int nC, int sign, double dt,
+                       double* __restrict__ sA_out, double* __restrict__ T_out, int p,
+                       int64_t inst_stride) {
+  constexpr int NE = DIM * (DIM + 1);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nC) return;
+  const int64_t off = (int64_t)blockIdx.y * inst_stride + (int64_t)c * NE;
+  double sA[NE], sAdt[NE], T[NE];
+#pragma unroll
+  for (int k = 0; k < NE; k++) {
+    double v = As[off + k];
+    if (sign == -1) v *= -1;
+    sA[k] = v;
+    sAdt[k] = v * dt;
+  }
+  if (DIM == 1) {
+    T[0] = exp(sAdt[0]);
+    T[1] = sAdt[1];
+    if (sA[0] != 0) T[1] = T[1] * (T[0] - 1) / sAdt[0];
+  } else if (DIM == 2) {
+    expm_affine2(sAdt, T, p);
+  } else {
+    expm_affine3(sAdt, T, p);
+  }
+#pragma unroll
+  for (int k = 0; k < NE; k++) {
+    if (sA_out) sA_out[off + k] = sA[k];
+    T_out[off + k] = T[k];
+  }
+}
+
+// The reference's gpu_expm(As, Ts, p) on an already scaled table.
+template <int DIM>
+__global__ void k_expm(const double* __restrict__ As, double* __restrict__ Ts, int nC, int p) {
+  constexpr int NE = DIM * (DIM + 1);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nC) return;
+  double A[NE], T[NE];
+#pragma unroll
+  for (int k = 0; k < NE; k++) A[k] = As[(int64_t)c * NE + k];
+  if (DIM == 2) expm_affine2(A, T, p);
+  else expm_affine3(A, T, p);
+#pragma unroll
+  for (int k = 0; k < NE; k++) Ts[(int64_t)c * NE + k] = T[k];
+}
+
+// ------------------------------------------------------------- helpers
+__device__ __forceinline__ int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0)); }
+
+__device__ __forceinline__ uint64_t ballot(bool b) { return __ballot(b); }
+
+__device__ __forceinline__ int rank_in(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+}
+
+template <int DIM>
+__device__ __forceinline__ void load_pt(const double* __restrict__ src, int64_t i, double* p) {
+  if (DIM == 2) {
+    const double2 v = *reinterpret_cast<const double2*>(src + i * 2);
+    p[0] = v.x;
+    p[1] = v.y;
+  } else {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) p[d] = src[i * DIM + d];
+  }
+}
+
+template <int DIM>
+__device__ __forceinline__ void store_pt(double* __restrict__ dst, int64_t i, const double* p) {
+  if (DIM == 2) {
+    *reinterpret_cast<double2*>(dst + i * 2) = make_double2(p[0], p[1]);
+  } else {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) dst[i * DIM + d] = p[d];
+  }
+}
+
+// Copy the Trel and A tables into LDS (both when they fit).
+template <int NE>
+__device__ __forceinline__ void stage_tables(double* lds, const double* __restrict__ Trels,
+                                             const double* __restrict__ As, int nC) {
+  const int n = nC * NE;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    lds[i] = Trels ? Trels[i] : 0.0;
+    lds[n + i] = As[i];
+  }
+  __syncthreads();
+}
+
+// --------------------------------------------------------- integration
+// MODE 0: calc_T (write the end point).  MODE 1: calc_trajectory (write the
+// point after every step into traj[(t*nPts + i)*DIM]).
+//
+// Lane state: p (current point), c (cell of p), t (steps done),
+// k (RK2 substeps still to run for the current step; 0 = next trip is a
+// Trel proposal).  One trip:
+//   k == 0: y = Trel_c [p;1]; if cell(y) == c accept (t++, c unchanged) else
+//           reject (k = nStepsODE, p and c unchanged)          (:389-401)
+//   k >  0: y = RK2 substep of A_c from p (solveODE's body, :179-194);
+//           p = y, c = cell(y); k--; on k == 0 the step is done (t++).
+// solveODE recomputes cell(p) at the top of every substep; here it is the
+// lookup at the end of the previous trip, of the same p: same value.
+template <int DIM, int TESS, bool POW2, bool LDS, int MODE>
+__global__ void __launch_bounds__(1024)
+    k_integrate(const double* __restrict__ pos0, double* __restrict__ out,
+                const double* __restrict__ Trels, const double* __restrict__ As, double h,
+                int nT, int nS, int64_t nPts, Geom g, int nC, int ptsPerWave,
+                int32_t* __restrict__ nfb_out) {
+  constexpr int NE = DIM * (DIM + 1);
+  extern __shared__ double lds[];
+  const double* Ttab = Trels;
+  const double* Atab = As;
+  if (LDS) {
+    stage_tables<NE>(lds, Trels, As, nC);
+    Ttab = lds;
+    Atab = lds + nC * NE;
+  }
+  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t begin = wave * (int64_t)ptsPerWave;
+  if (begin >= nPts) return;
+  const int64_t end = begin + ptsPerWave < nPts ? begin + ptsPerWave : nPts;
+  int64_t next = begin;  // wave-uniform cursor into [begin, end)
+
+  double p[DIM];
+  int c = 0, t = 0, k = 0, nfb = 0;
+  int64_t my = 0;
+  bool active = false;
+  for (;;) {
+    if (active && t >= nT) {  // retire a finished point
+      if (MODE == 0) store_pt<DIM>(out, my, p);
+      if (nfb_out) nfb_out[my] = nfb;
+      active = false;
+    }
+    const uint64_t idle = ballot(!active);
+    if (idle != 0 && next < end) {  // refill idle lanes from the wave's range
+      if (!active) {
+        const int64_t cand = next + rank_in(idle);
+        if (cand < end) {
+          my = cand;
+          active = true;
+          load_pt<DIM>(pos0, my, p);
+          c = cell_of<DIM, TESS, POW2>(p, g);
+          t = 0;
+          k = 0;
+          nfb = 0;
+        }
+      }
+      next += __popcll(idle);
+    }
+    if (ballot(active) == 0) break;
+    if (!active || t >= nT) continue;
+
+    const double* M = (k == 0 ? Ttab : Atab) + c * NE;
+    double y[DIM];
+    affine<DIM>(y, M, p);
+    if (k != 0) {  // RK2 midpoint substep (solveODE, calc_2Dto2D.cu:182-194)
+      double m[DIM];
+#pragma unroll
+      for (int d = 0; d < DIM; d++) m[d] = p[d] + h * y[d] / 2.;
+      affine<DIM>(y, M, m);
+#pragma unroll
+      for (int d = 0; d < DIM; d++) y[d] = p[d] + y[d] * h;
+    }
+    const int cy = cell_of<DIM, TESS, POW2>(y, g);
+    bool stepped = false;
+    if (k == 0) {
+      if (cy == c) {
+#pragma unroll
+        for (int d = 0; d < DIM; d++) p[d] = y[d];
+        stepped = true;
+      } else {
+        nfb++;
+        if (nS > 0) k = nS;
+        else stepped = true;  // zero substeps: the step leaves p where it was
+      }
+    } else {
+#pragma unroll
+      for (int d = 0; d < DIM; d++) p[d] = y[d];
+      c = cy;
+      if (--k == 0) stepped = true;
+    }
+    if (stepped) {
+      if (MODE == 1) store_pt<DIM>(out, (int64_t)t * nPts + my, p);
+      t++;
+    }
+  }
+}
+
+// calc_T_simple: nStepsODE * nTimeSteps RK2 substeps (calc_2Dto2D.cu:412-445).
+template <int DIM, int TESS, bool POW2, bool LDS>
+__global__ void __launch_bounds__(1024)
+    k_simple(const double* __restrict__ pos0, double* __restrict__ out, const double* __restrict__ As,
+             double h, int nSteps, int64_t nPts, Geom g, int nC) {
+  constexpr int NE = DIM * (DIM + 1);
+  extern __shared__ double lds[];
+  const double* Atab = As;
+  if (LDS) {
+    for (int i = threadIdx.x; i < nC * NE; i += blockDim.x) lds[i] = As[i];
+    __syncthreads();
+    Atab = lds;
+  }
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nPts) return;
+  double p[DIM], y[DIM], m[DIM];
+  load_pt<DIM>(pos0, i, p);
+  for (int s = 0; s < nSteps; s++) {
+    const double* A = Atab + cell_of<DIM, TESS, POW2>(p, g) * NE;
+    affine<DIM>(y, A, p);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) m[d] = p[d] + h * y[d] / 2.;
+    affine<DIM>(y, A, m);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) p[d] += y[d] * h;
+  }
+  store_pt<DIM>(out, i, p);
+}
+
+template <int DIM, int TESS, bool POW2, bool LDS>
+__global__ void __launch_bounds__(1024)
+    k_velocity(const double* __restrict__ pts, double* __restrict__ vel, const double* __restrict__ As,
+               int64_t nPts, Geom g, int nC) {
+  constexpr int NE = DIM * (DIM + 1);
+  extern __shared__ double lds[];
+  const double* Atab = As;
+  if (LDS) {
+    for (int i = threadIdx.x; i < nC * NE; i += blockDim.x) lds[i] = As[i];
+    __syncthreads();
+    Atab = lds;
+  }
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nPts) return;
+  double p[DIM], v[DIM];
+  load_pt<DIM>(pts, i, p);
+  affine<DIM>(v, Atab + cell_of<DIM, TESS, POW2>(p, g) * NE, p);
+  store_pt<DIM>(vel, i, v);
+}
+
+template <int DIM, int TESS, bool POW2>
+__global__ void k_cell_idx(const double* __restrict__ pts, int32_t* __restrict__ cell, int64_t nPts, Geom g) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nPts) return;
+  double p[DIM];
+  load_pt<DIM>(pts, i, p);
+  cell[i] = cell_of<DIM, TESS, POW2>(p, g);
+}
+
+// ---------------------------------------------------- remap + likelihood
+// Bilinear sample, zero outside (OpenCV INTER_LINEAR, BORDER_CONSTANT 0),
+// weights from the double coordinates' fractional parts, float arithmetic.
+__device__ __forceinline__ float sample2(const float* __restrict__ img, int H, int W, double x, double y) {
+  const double xf = floor(x), yf = floor(y);
+  // a full pixel or more outside, or not a number: zero
+  if (!(xf >= -1.0 && xf < (double)W && yf >= -1.0 && yf < (double)H)) return 0.f;
+  const float fx = (float)(x - xf), fy = (float)(y - yf);
+  float v00 = 0.f, v01 = 0.f, v10 = 0.f, v11 = 0.f;
+  const int x0 = (int)xf, y0 = (int)yf, x1 = x0 + 1, y1 = y0 + 1;
+  const bool okx0 = x0 >= 0, okx1 = x1 < W, oky0 = y0 >= 0, oky1 = y1 < H;
+  if (oky0 && okx0) v00 = img[(int64_t)y0 * W + x0];
+  if (oky0 && okx1) v01 = img[(int64_t)y0 * W + x1];
+  if (oky1 && okx0) v10 = img[(int64_t)y1 * W + x0];
+  if (oky1 && okx1) v11 = img[(int64_t)y1 * W + x1];
+  const float gx = 1.f - fx, gy = 1.f - fy;
+  const float a = gx * v00 + fx * v01;
+  const float b = gx * v10 + fx * v11;
+  return gy * a + fy * b;
+}
+
+__device__ __forceinline__ float sample3(const float* __restrict__ vol, int D, int H, int W, double x,
+                                         double y, double z) {
+  const double xf = floor(x), yf = floor(y), zf = floor(z);
+  if (!(xf >= -1.0 && xf < (double)W && yf >= -1.0 && yf < (double)H && zf >= -1.0 && zf < (double)D))
+    return 0.f;
+  const float fx = (float)(x - xf), fy = (float)(y - yf), fz = (float)(z - zf);
+  float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const int x0 = (int)xf, y0 = (int)yf, z0 = (int)zf;
+#pragma unroll
+  for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+      for (int dx = 0; dx < 2; dx++) {
+        const int xx = x0 + dx, yy = y0 + dy, zz = z0 + dz;
+        if (xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D)
+          v[dz * 4 + dy * 2 + dx] = vol[((int64_t)zz * H + yy) * W + xx];
+      }
+  const float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
+  const float a0 = gx * v[0] + fx * v[1];
+  const float a1 = gx * v[2] + fx * v[3];
+  const float a2 = gx * v[4] + fx * v[5];
+  const float a3 = gx * v[6] + fx * v[7];
+  const float b0 = gy * a0 + fy * a1;
+  const float b1 = gy * a2 + fy * a3;
+  return gz * b0 + fz * b1;
+}
+
+__global__ void k_remap2d(const double* __restrict__ pts, const float* __restrict__ img,
+                          float* __restrict__ out, int H, int W) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)H * W) return;
+  const double2 q = *reinterpret_cast<const double2*>(pts + 2 * i);
+  out[i] = sample2(img, H, W, q.x, q.y);
+}
+
+__global__ void k_remap3d(const double* __restrict__ pts, const float* __restrict__ vol,
+                          float* __restrict__ out, int D, int H, int W) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)D * H * W) return;
+  out[i] = sample3(vol, D, H, W, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+}
+
+// Block sum of a per-lane double through LDS; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < nw; i++) s += red[i];
+  return s;
+}
+
+// Per-block partial sums of (x - y)^2 over float images; the difference and
+// its square are taken in double (two FP64 ops per pixel, and the sum of
+// 10^5 terms keeps ~1e-15 relative accuracy instead of ~1e-7).
+__global__ void k_ssd_partial(const float* __restrict__ x, const float* __restrict__ y, int64_t n,
+                              double* __restrict__ partial) {
+  __shared__ double red[16];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double e = (double)x[i] - (double)y[i];
+    acc += e * e;
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// ll = -0.5 / sigma^2 * sum(partials), one wave (gaussian.py:30-34 + sum).
+__global__ void k_reduce_ll(const double* __restrict__ partial, int n, double sigma, double* __restrict__ ll) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 64) acc += partial[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (threadIdx.x == 0) *ll = acc * (-0.5 / (sigma * sigma));
+}
+
+}  // namespace cpab

@@ -1,0 +1,135 @@
+/*
+ * cpab_hip.h -- C ABI of the MI355X (gfx950) CPA-velocity integration library
+ * (libcpab_hip.so).
+ *
+ * Drop-in for the reference's GPU backend object `Calcs`
+ * (cpab/gpu/Calcs.py:44-1024) and its expm helper `gpu_expm`
+ * (cpab/gpu/expm/expm_affine_by_series_2D.py:246-258): same kernel
+ * arguments, precompiled HIP kernels instead of a pycuda JIT.
+ *
+ * Conventions
+ *  - Every array argument is a DEVICE pointer owned by the caller (the
+ *    reference writes into the caller's `out` buffers the same way).
+ *  - Points are AoS, row-major (nPts, dim) float64: pts[i*dim + d]
+ *    (cpab/gpu/calc_2Dto2D.cu:380-381).  Per-cell tables are (nC, dim*(dim+1))
+ *    row-major float64, i.e. the top dim rows of each affine matrix
+ *    (cpab/cpaNd/_PAT.py:31, _CpaCalcs.py:243).  Cell ids are int32 (nPts,).
+ *  - dim is 1, 2 or 3; tess is 1 (type I: triangles / tetrahedra) or 2
+ *    (type II: intervals / rectangles / boxes), the reference's TESS_TYPE
+ *    (cpab/gpu/Calcs.py:161).  Type I needs dim 2 or 3.
+ *  - nC0, nC1, nC2, inc_x, inc_y, inc_z are what Calcs.parse_nCs_and_incs
+ *    passes (cpab/gpu/Calcs.py:215-233): unused counts 0, unused increments 1.
+ *  - Work is enqueued on the context's stream; no call synchronises unless
+ *    it says so.  Functions return CPAB_OK or an error code and never throw;
+ *    cpab_last_error() describes the last failure of the context.
+ *  - A context is used by one host thread at a time; distinct contexts are
+ *    independent.
+ */
+#ifndef CPAB_HIP_H
+#define CPAB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cpab_ctx cpab_ctx;
+
+enum {
+  CPAB_OK = 0,
+  CPAB_ERR_ARG = 1,         /* bad shape / value: the reference's ValueError */
+  CPAB_ERR_UNSUPPORTED = 2, /* e.g. tess I in 1D: NotImplementedError */
+  CPAB_ERR_ALIAS = 3,       /* input and output share memory (_CpaCalcs.py:382-389) */
+  CPAB_ERR_HIP = 4,         /* a HIP runtime call failed */
+  CPAB_ERR_NOMEM = 5
+};
+
+/* Context on `device` using the default stream.  Replaces Calcs.__init__
+ * (cpab/gpu/Calcs.py:44-63), which JIT-compiled the kernels. */
+int cpab_create(int device, cpab_ctx **out);
+int cpab_destroy(cpab_ctx *ctx);
+const char *cpab_last_error(const cpab_ctx *ctx);
+/* Enqueue subsequent work on `hip_stream` (a hipStream_t; NULL = default). */
+int cpab_set_stream(cpab_ctx *ctx, void *hip_stream);
+int cpab_synchronize(cpab_ctx *ctx);
+/* Library / build identification string. */
+const char *cpab_version(void);
+
+/* Fused sign, dt scaling and per-cell matrix exponential:
+ *   signedAs = sign * As;  Trels = expm(dt * signedAs)   (closed form in 1D).
+ * Replaces CpaCalcs.prepare_signedSqAs_and_Tlocals_for_gpu
+ * (cpab/cpaNd/_CpaCalcs.py:227-313) with its gpu_expm call.
+ * sign is +1 (calc_T_fwd) or -1 (calc_T_inv).  signedAs_out may be NULL. */
+int cpab_prep_trels(cpab_ctx *ctx, int dim, int nC, const double *As, int sign, double dt,
+                    double *signedAs_out, double *Trels_out);
+
+/* gpu_expm(As, Ts, p): series / closed-form expm of an already scaled table
+ * (cpab/gpu/expm/expm_affine_by_series_2D.py:246, _3D.py:228).  dim 2 or 3. */
+int cpab_expm(cpab_ctx *ctx, int dim, int nC, const double *As, double *Ts, int p);
+
+/* calc_T: Calcs.calc_T (cpab/gpu/Calcs.py:454-561) and its kernel
+ * (cpab/gpu/calc_2Dto2D.cu:348-408, calc_1Dto1D.cu:485-548,
+ * calc_3Dto3D.cu:353-416).  Trels/As are the tables of cpab_prep_trels.
+ * n_fallback (optional, int32 (nPts,)) receives, per point, the number of
+ * steps whose Trel proposal left its cell and were redone by RK2. */
+int cpab_calc_T(cpab_ctx *ctx, int dim, int tess, int nC, const double *pos0, double *pos,
+                const double *Trels, const double *As, double dt, int nTimeSteps, int nStepsODE,
+                int64_t nPts, int nC0, int nC1, int nC2, double inc_x, double inc_y,
+                double inc_z, int32_t *n_fallback);
+
+/* calc_T_simple: Calcs.calc_T_simple (cpab/gpu/Calcs.py:339-450), kernel
+ * cpab/gpu/calc_2Dto2D.cu:412-445 (nStepsODE*nTimeSteps RK2 substeps). */
+int cpab_calc_T_simple(cpab_ctx *ctx, int dim, int tess, int nC, const double *pos0, double *pos,
+                       const double *As, double dt, int nTimeSteps, int nStepsODE, int64_t nPts,
+                       int nC0, int nC1, int nC2, double inc_x, double inc_y, double inc_z);
+
+/* calc_trajectory: Calcs.calc_trajectory (cpab/gpu/Calcs.py:735-814), kernel
+ * cpab/gpu/calc_2Dto2D.cu:499-556.  traj is (nTimeSteps, nPts, dim): row t is
+ * the position after t+1 steps (the host prepends pos0, _CpaCalcs.py:866-868). */
+int cpab_calc_trajectory(cpab_ctx *ctx, int dim, int tess, int nC, const double *pos0,
+                         double *traj, const double *Trels, const double *As, double dt,
+                         int nTimeSteps, int nStepsODE, int64_t nPts, int nC0, int nC1, int nC2,
+                         double inc_x, double inc_y, double inc_z);
+
+/* calc_v: Calcs.calc_velocities (cpab/gpu/Calcs.py:816-924), kernel
+ * cpab/gpu/calc_2Dto2D.cu:561-602.  As is the unsigned table. */
+int cpab_calc_v(cpab_ctx *ctx, int dim, int tess, int nC, const double *pts, double *vel,
+                const double *As, int64_t nPts, int nC0, int nC1, int nC2, double inc_x,
+                double inc_y, double inc_z);
+
+/* calc_cell_idx: Calcs.calc_cell_idx (cpab/gpu/Calcs.py:968-1010), kernel
+ * cpab/gpu/calc_2Dto2D.cu:323-344. */
+int cpab_calc_cell_idx(cpab_ctx *ctx, int dim, int tess, const double *pts, int32_t *cell_idx,
+                       int64_t nPts, int nC0, int nC1, int nC2, double inc_x, double inc_y,
+                       double inc_z);
+
+/* remap_fwd / remap_inv of an H x W float image: out[r*W + c] = img(pts[r*W + c]),
+ * bilinear, zero outside (cpab/cpa2d/TransformWrapper.py:261-317; the
+ * reference delegates to the un-vendored of.gpu.remap.remap2d). */
+int cpab_remap2d(cpab_ctx *ctx, const double *pts, const float *img, float *out, int H, int W);
+
+/* Trilinear remap of a D x H x W float volume (cpab/cpa3d/TransformWrapper.py:254-290). */
+int cpab_remap3d(cpab_ctx *ctx, const double *pts, const float *vol, float *out, int D, int H,
+                 int W);
+
+/* Gaussian SSD log-likelihood sum_i -0.5*(x_i - y_i)^2/sigma^2, written to the
+ * device scalar *ll (cpab/cpa2d/model/transformations/register/gpu/gaussian.py:15-34
+ * and the sum of _ScaleDependentLogLikelihoodGeneral.py:118). */
+int cpab_ssd_ll(cpab_ctx *ctx, const float *x, const float *y, int64_t n, double sigma,
+                double *ll);
+
+/* Kernel timing with HIP events on the context's stream (no reference
+ * counterpart; the reference wraps its launches in GpuTimer,
+ * cpab/gpu/Calcs.py:537-563).  While enabled, every integration launch
+ * (calc_T, trajectory, simple) is bracketed by events; cpab_prof_read
+ * synchronises, returns the launch count and summed milliseconds, and
+ * resets the counters. */
+int cpab_prof_enable(cpab_ctx *ctx, int on);
+int cpab_prof_read(cpab_ctx *ctx, int64_t *n_launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CPAB_HIP_H */

@@ -1,0 +1,304 @@
+"""GPU parity of the HIP kernels against the CPU oracle, through the C ABI.
+
+Bar: cell ids, velocities, integrated points, trajectories, remaps are
+bit-identical to the oracle when both start from the same per-cell tables
+(same IEEE operations in the same order).  The device matrix exponential
+uses the GPU's libm (exp/cosh/sinh/cos/sin), so end-to-end results that go
+through it are compared at 1e-10 except for points
+whose trajectory passed within 1e-12 of a facet, which are counted.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from cpabdiffeo_amd import configs
+from cpabdiffeo_amd.calcs import Calcs, gpu_prep_trels
+from cpabdiffeo_amd.space import CpaSpace
+from cpabdiffeo_amd.tessellation import Tessellation
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def dev(torch, x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def host(t):
+    t = t.cpu().numpy()
+    return t
+
+
+GEOMS = [
+    ("2D-I-4x4", [0, 0], [1, 1], (4, 4), "I"),
+    ("2D-I-4x4-px512", [0, 0], [512, 512], (4, 4), "I"),
+    ("2D-I-3x3-nonpow2", [0, 0], [1, 1], (3, 3), "I"),
+    ("2D-I-16x16", [0, 0], [1, 1], (16, 16), "I"),
+    ("2D-II-4x4", [0, 0], [1, 1], (4, 4), "II"),
+    ("2D-II-5x3-nonpow2", [0, 0], [1, 1], (5, 3), "II"),
+    ("1D-50", [0], [1], (50,), "II"),
+    ("1D-8", [0], [1], (8,), "II"),
+    ("3D-I-4x4x4", [0, 0, 0], [1, 1, 1], (4, 4, 4), "I"),
+    ("3D-I-2x2x2-px64", [0, 0, 0], [64, 64, 64], (2, 2, 2), "I"),
+    ("3D-II-2x3x4", [0, 0, 0], [1, 1, 1], (2, 3, 4), "II"),
+]
+
+
+def lookup_points(T, rng, n=20000):
+    dim = T.dim_domain
+    ext = T.XMAXS
+    pts = [rng.uniform(-0.2, 1.2, (n, dim)) * ext]
+    # exact cell walls and diagonals: multiples of the increments
+    k = rng.integers(0, 4 * max(T.nCs) + 1, (n // 4, dim))
+    pts.append(k * (T.incs / 4))
+    special = np.array([np.nan, np.inf, -np.inf, 1e300, -1e300, 0.0, -0.0, 1e-310, -1e-310])
+    pts.append(rng.choice(special, (n // 10, dim)))
+    return np.concatenate(pts).astype(np.float64)
+
+
+@pytest.mark.parametrize("name,lo,hi,nCs,tess", GEOMS)
+def test_cell_idx_bitexact(gpu, orc, name, lo, hi, nCs, tess):
+    torch = gpu
+    say("cell_idx", name)
+    T = Tessellation(lo, hi, nCs, tess)
+    rng = np.random.default_rng(20)
+    pts = lookup_points(T, rng)
+    want = orc.calc_cell_idx(T.geom(), pts)
+    c = Calcs(T.nC, np.float64, T.dim_domain, T.dim_domain, "I" if T.type == 1 else "II")
+    out = torch.zeros(len(pts), dtype=torch.int32, device="cuda")
+    c.calc_cell_idx(None, None, dev(torch, pts), out, T.dim_domain, T.dim_domain, T.nCs, T.incs)
+    got = host(out)
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, (name, pts[bad[:5]], got[bad[:5]], want[bad[:5]])
+
+
+def space_for(lo, hi, nCs, tess):
+    dim = len(lo)
+    if dim == 1:
+        return CpaSpace(lo, hi, list(nCs), zero_v_across_bdry=[True], tess="II")
+    if tess == "I":
+        return CpaSpace(lo, hi, list(nCs), zero_v_across_bdry=[True] * dim, tess="I", valid_outside=False)
+    return CpaSpace(lo, hi, list(nCs), zero_v_across_bdry=[False] * dim, tess="II")
+
+
+def random_field(sp, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    ext = float(np.max(sp.XMAXS))
+    Av = sp.B @ rng.normal(size=sp.d)
+    A = Av.reshape(sp.nC, sp.dim_domain, -1)
+    A[:, :, -1] *= ext  # velocities of order ext per unit time
+    return (A * scale).reshape(-1)
+
+
+@pytest.mark.parametrize("name,lo,hi,nCs,tess", GEOMS)
+def test_calc_v_bitexact(gpu, orc, name, lo, hi, nCs, tess):
+    torch = gpu
+    say("calc_v", name)
+    sp = space_for(lo, hi, nCs, tess)
+    Av = random_field(sp, 21)
+    pts = lookup_points(sp.tessellation, np.random.default_rng(22), 5000)
+    pts = pts[np.isfinite(pts).all(1) & (np.abs(pts) < 1e6).all(1)]
+    want = orc.calc_v(sp.tessellation.geom(), pts, Av.reshape(sp.nC, -1))
+    sp.update_pat(Av)
+    out = torch.zeros(pts.shape, dtype=torch.float64, device="cuda")
+    sp.calc_v(pts=dev(torch, pts), out=out)
+    got = host(out)
+    assert np.array_equal(got, want), np.abs(got - want).max()
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("sign", [1, -1])
+def test_prep_matches_oracle(gpu, orc, dim, sign):
+    torch = gpu
+    say("prep", dim, sign)
+    rng = np.random.default_rng(23 + dim)
+    ne = dim * (dim + 1)
+    As = rng.normal(size=(500, ne))
+    As[:50, :] = 0.0                      # zero field: series branch, identity
+    if dim == 2:
+        As[50:100, 3:5] = As[50:100, 0:2] * 0.5  # singular linear part: series branch
+        As[100:110] = [0.3, 0.0, 0.2, 0.0, 0.3, -0.1]  # repeated eigenvalue
+    if dim == 1:
+        As[100:150, 0] = 0.0              # a == 0: Trel is a translation
+    dt = 0.01
+    sA_w, T_w = orc.prep(dim, As, sign, dt)
+    sA = torch.zeros(As.shape, dtype=torch.float64, device="cuda")
+    T = torch.zeros(As.shape, dtype=torch.float64, device="cuda")
+    gpu_prep_trels(dim, dev(torch, As), sign, dt, sA, T)
+    assert np.array_equal(host(sA), sA_w)
+    Tg = host(T)
+    # rows computed with +,-,*,/ only (series branch, 1D translations) are bit-exact
+    if dim == 2:
+        assert np.array_equal(Tg[:100], T_w[:100])
+    if dim == 3:
+        assert np.array_equal(Tg, T_w)
+    if dim == 1:
+        assert np.array_equal(Tg[:50], T_w[:50]) and np.array_equal(Tg[100:150], T_w[100:150])
+    # Rows that go through exp/cosh/sinh/cos/sin differ by the libm ulp; the
+    # closed-form translation (T00 - 1) * a + T01 * b with a = A^{-1} b
+    # amplifies that by the conditioning of the linear part (entries are O(1)).
+    assert np.abs(Tg - T_w).max() <= 1e-13
+
+
+INTEG = [g for g in GEOMS if g[0] != "2D-I-16x16"] + [("2D-I-16x16", [0, 0], [1, 1], (16, 16), "I")]
+
+
+@pytest.mark.parametrize("name,lo,hi,nCs,tess", INTEG)
+def test_calc_T_bitexact_given_tables(gpu, orc, name, lo, hi, nCs, tess):
+    """Same per-cell tables in, bit-identical points and fallback counts out."""
+    torch = gpu
+    say("calc_T", name)
+    sp = space_for(lo, hi, nCs, tess)
+    g = sp.tessellation.geom()
+    Av = random_field(sp, 24)
+    rng = np.random.default_rng(25)
+    pts = rng.uniform(0, 1, (20000, sp.dim_domain)) * sp.XMAXS
+    for sign in (1, -1):
+        sA, T = orc.prep(sp.dim_domain, Av.reshape(sp.nC, -1), sign, 0.01)
+        want, nfb_w, _ = orc.calc_T(g, pts, T, sA, 0.01, 100, 10, probe=True)
+        out = torch.zeros(pts.shape, dtype=torch.float64, device="cuda")
+        nfb = torch.zeros(len(pts), dtype=torch.int32, device="cuda")
+        sp._gpu_calcs.calc_T(None, None, dev(torch, T), dev(torch, sA), dev(torch, pts), 0.01, 100, 10, out,
+                             sp.dim_domain, sp.dim_range, sp.nCs, sp.incs, n_fallback=nfb)
+        got = host(out)
+        bad = np.nonzero((got != want).any(1))[0]
+        assert bad.size == 0, (name, sign, bad.size, np.abs(got - want).max())
+        assert np.array_equal(host(nfb), nfb_w)
+        assert nfb_w.sum() > 0 or name.startswith("1D")  # the fallback path was exercised
+
+
+def test_fixture_cfg1_small(gpu):
+    torch = gpu
+    say("fixture cfg1_small")
+    f = np.load(os.path.join(HERE, "golden", "cfg1_small.npz"))
+    sp = configs.unit_space((4, 4), "I")
+    for name in ("fwd", "inv"):
+        out = torch.zeros(f["pts"].shape, dtype=torch.float64, device="cuda")
+        nfb = torch.zeros(len(f["pts"]), dtype=torch.int32, device="cuda")
+        sp._gpu_calcs.calc_T(None, None, dev(torch, f["Trels_" + name]), dev(torch, f["signedAs_" + name]),
+                             dev(torch, f["pts"]), float(f["dt"]), int(f["nTimeSteps"]), int(f["nStepsODE"]),
+                             out, 2, 2, f["nCs"], f["incs"], n_fallback=nfb)
+        assert np.array_equal(host(out), f["T_" + name])
+        assert np.array_equal(host(nfb), f["nfb_" + name])
+
+
+def test_config1_full_path(gpu, orc):
+    """Device prep + calc_T on the full 512x512 config-1 grid vs the oracle."""
+    torch = gpu
+    say("config1 full path 262144 points")
+    cfg = configs.config1()
+    sp = cfg["space"]
+    g = sp.tessellation.geom()
+    sp.update_pat(cfg["Avees"])
+    pts = cfg["pts"]
+    for sign in (1, -1):
+        sA, T = orc.prep(2, cfg["Avees"].reshape(sp.nC, -1), sign, cfg["dt"])
+        want, nfb_w, md = orc.calc_T(g, pts, T, sA, cfg["dt"], 100, 10, probe=True)
+        out = torch.zeros(pts.shape, dtype=torch.float64, device="cuda")
+        sp._calc_T(pts=dev(torch, pts), mysign=sign, out=out, dt=cfg["dt"], nTimeSteps=100, nStepsODEsolver=10)
+        got = host(out)
+        diff = np.abs(got - want).max(1)
+        near = md < 1e-12
+        say("sign", sign, "max diff", diff.max(), "points off by >1e-10:", int((diff > 1e-10).sum()),
+            "near-facet points:", int(near.sum()), "bit-identical:", int((diff == 0).sum()))
+        assert diff[~near].max() <= 1e-10
+
+
+def test_trajectory_and_simple_bitexact(gpu, orc):
+    torch = gpu
+    say("trajectory + simple")
+    cfg = configs.config1(grid=64)
+    sp = cfg["space"]
+    g = sp.tessellation.geom()
+    pts = cfg["pts"]
+    sA, T = orc.prep(2, cfg["Avees"].reshape(sp.nC, -1), 1, cfg["dt"])
+    want = orc.calc_trajectory(g, pts, T, sA, cfg["dt"], 100, 10)
+    calcs = sp._gpu_calcs
+    traj = calcs.calc_trajectory(None, None, dev(torch, T), dev(torch, sA), dev(torch, pts), cfg["dt"], 100, 10,
+                                 2, 2, sp.nCs, sp.incs)
+    assert np.array_equal(host(traj), want)
+    out = torch.zeros(pts.shape, dtype=torch.float64, device="cuda")
+    calcs.calc_T(None, None, dev(torch, T), dev(torch, sA), dev(torch, pts), cfg["dt"], 100, 10, out, 2, 2,
+                 sp.nCs, sp.incs)
+    assert np.array_equal(host(traj)[-1], host(out))
+    ws = orc.calc_T_simple(g, pts, sA, cfg["dt"], 100, 10)
+    calcs.calc_T_simple(None, None, dev(torch, sA), dev(torch, pts), cfg["dt"], 100, 10, out, 2, 2, sp.nCs, sp.incs)
+    assert np.array_equal(host(out), ws)
+
+
+def test_scheduling_does_not_change_results(gpu, orc):
+    """More points per lane (the refill path) gives the same bits."""
+    torch = gpu
+    say("points-per-lane sweep")
+    cfg = configs.config1(grid=128)
+    sp = cfg["space"]
+    sA, T = orc.prep(2, cfg["Avees"].reshape(sp.nC, -1), 1, cfg["dt"])
+    want = orc.calc_T(sp.tessellation.geom(), cfg["pts"], T, sA, cfg["dt"], 100, 10)
+    for ppl in ("1", "3", "16"):
+        os.environ["CPAB_PTS_PER_LANE"] = ppl
+        try:
+            out = torch.zeros(cfg["pts"].shape, dtype=torch.float64, device="cuda")
+            sp._gpu_calcs.calc_T(None, None, dev(torch, T), dev(torch, sA), dev(torch, cfg["pts"]), cfg["dt"], 100,
+                                 10, out, 2, 2, sp.nCs, sp.incs)
+            assert np.array_equal(host(out), want), ppl
+        finally:
+            del os.environ["CPAB_PTS_PER_LANE"]
+
+
+def test_remap_and_ssd(gpu, orc):
+    torch = gpu
+    say("remap2d/3d + ssd")
+    from cpabdiffeo_amd.calcs import context_for
+    rng = np.random.default_rng(26)
+    img = configs.blob_image(96, seed=1)
+    pts = rng.uniform(-3, 99, (96 * 96, 2))
+    want = orc.remap2d(pts, img)
+    out = torch.zeros(img.shape, dtype=torch.float32, device="cuda")
+    ti, tp = dev(torch, img), dev(torch, pts)
+    ctx = context_for(ti)
+    ctx.call("cpab_remap2d", tp.data_ptr(), ti.data_ptr(), out.data_ptr(), 96, 96)
+    assert np.array_equal(host(out), want)
+    vol = configs.blob_image(20, seed=2, dims=3)
+    p3 = rng.uniform(-2, 22, (20 ** 3, 3))
+    w3 = orc.remap3d(p3, vol)
+    o3 = torch.zeros(vol.shape, dtype=torch.float32, device="cuda")
+    tv, tp3 = dev(torch, vol), dev(torch, p3)
+    ctx.call("cpab_remap3d", tp3.data_ptr(), tv.data_ptr(), o3.data_ptr(), 20, 20, 20)
+    assert np.array_equal(host(o3), w3)
+    ll = torch.zeros(1, dtype=torch.float64, device="cuda")
+    y = configs.blob_image(96, seed=3)
+    ctx.call("cpab_ssd_ll", out.data_ptr(), dev(torch, y).data_ptr(), 96 * 96, 0.02, ll.data_ptr())
+    w = orc.ssd_ll(want, y, 0.02)
+    assert abs(host(ll)[0] - w) <= 1e-12 * abs(w)
+
+
+def test_errors_and_edge_sizes(gpu):
+    torch = gpu
+    say("errors and edge sizes")
+    sp = configs.unit_space((4, 4), "I")
+    c = sp._gpu_calcs
+    tab = torch.zeros((sp.nC, 6), dtype=torch.float64, device="cuda")
+    tab[:, 0] = 1.0
+    tab[:, 4] = 1.0
+    pts = torch.rand((100, 2), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):  # aliasing (cpab/cpaNd/_CpaCalcs.py:382-389)
+        c.calc_T(None, None, tab, tab, pts, 0.01, 10, 10, pts, 2, 2, sp.nCs, sp.incs)
+    with pytest.raises(ValueError):  # nC does not match the tessellation
+        Calcs(10, np.float64, 2, 2, "I").calc_T(None, None, tab[:10], tab[:10], pts, 0.01, 10, 10,
+                                                torch.zeros_like(pts), 2, 2, sp.nCs, sp.incs)
+    with pytest.raises(NotImplementedError):
+        Calcs(64, np.float32, 2, 2, "I")
+    empty = torch.zeros((0, 2), dtype=torch.float64, device="cuda")
+    c.calc_T(None, None, tab, tab, empty, 0.01, 10, 10, torch.zeros_like(empty), 2, 2, sp.nCs, sp.incs)
+    out = torch.zeros_like(pts)
+    c.calc_T(None, None, tab, tab, pts, 0.01, 0, 10, out, 2, 2, sp.nCs, sp.incs)
+    assert torch.equal(out, pts)  # zero steps: a copy
+    c.calc_T(None, None, tab, tab, pts, 0.01, 10, 0, out, 2, 2, sp.nCs, sp.incs)  # zero substeps terminates
+    torch.cuda.synchronize()

@@ -1,0 +1,66 @@
+"""Pins of the oracle's remap and SSD likelihood.
+
+The reference delegates remap to the un-vendored `of` package (parity
+unpinned there); this project defines remap as OpenCV INTER_LINEAR with a
+zero constant border, pinned here by:
+* identity maps reproduce the image bit for bit;
+* bilinear interpolation reproduces affine images exactly inside the grid;
+* scipy.ndimage.map_coordinates(order=1) agrees on interior points;
+* the half-pixel border blends with zero; far outside is zero.
+The SSD likelihood is pinned by closed forms on constant images.
+"""
+import numpy as np
+from scipy.ndimage import map_coordinates
+
+
+def test_identity_map_copies(orc):
+    rng = np.random.default_rng(0)
+    img = rng.uniform(size=(33, 47)).astype(np.float32)
+    yy, xx = np.mgrid[0:33, 0:47]
+    pts = np.stack([xx.ravel(), yy.ravel()], 1).astype(np.float64)
+    assert np.array_equal(orc.remap2d(pts, img), img)
+    vol = rng.uniform(size=(5, 6, 7)).astype(np.float32)
+    zz, yy, xx = np.mgrid[0:5, 0:6, 0:7]
+    p3 = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], 1).astype(np.float64)
+    assert np.array_equal(orc.remap3d(p3, vol), vol)
+
+
+def test_affine_image_exact_inside(orc):
+    H, W = 40, 50
+    yy, xx = np.mgrid[0:H, 0:W]
+    img = (0.1 + 0.01 * xx + 0.02 * yy).astype(np.float32)
+    rng = np.random.default_rng(1)
+    pts = np.stack([rng.uniform(0, W - 1, H * W), rng.uniform(0, H - 1, H * W)], 1)
+    out = orc.remap2d(pts, img).ravel()
+    want = 0.1 + 0.01 * pts[:, 0] + 0.02 * pts[:, 1]
+    np.testing.assert_allclose(out, want, rtol=2e-6, atol=2e-6)
+
+
+def test_matches_map_coordinates_inside(orc):
+    rng = np.random.default_rng(2)
+    img = rng.uniform(size=(30, 30)).astype(np.float32)
+    pts = rng.uniform(0, 29, (900, 2))
+    want = map_coordinates(img.astype(np.float64), [pts[:, 1], pts[:, 0]], order=1)
+    np.testing.assert_allclose(orc.remap2d(pts, img).ravel(), want, rtol=1e-5, atol=1e-6)
+
+
+def test_border_blends_with_zero(orc):
+    img = np.full((4, 4), 2.0, np.float32)
+    pts = np.zeros((16, 2))
+    pts[0] = (-0.5, 1.0)    # half a pixel left of column 0
+    pts[1] = (3.5, 2.0)     # half a pixel right of the last column
+    pts[2] = (-1.0, 1.0)    # a full pixel out: zero
+    pts[3] = (1.0, -3.0)    # far above
+    pts[4] = (np.nan, 1.0)
+    pts[5:] = (1.0, 1.0)
+    out = orc.remap2d(pts, img).ravel()
+    assert out[0] == 1.0 and out[1] == 1.0 and out[2] == 0.0 and out[3] == 0.0 and out[4] == 0.0
+    assert (out[5:] == 2.0).all()
+
+
+def test_ssd_closed_form(orc):
+    n = 1000
+    x = np.full(n, 0.75, np.float32)
+    y = np.full(n, 0.25, np.float32)
+    assert orc.ssd_ll(x, y, 0.02) == -0.5 * n * 0.25 / 0.0004
+    assert orc.ssd_ll(x, x, 0.02) == 0.0
