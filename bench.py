@@ -1,0 +1,210 @@
+"""Benchmark: device-resident MCMC image registration (one chain, 256 x 256).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload: 2D registration by Metropolis on a 256 x 256 FP32 blob image
+(40 Gaussian blobs, seed 0); the target is the source warped by a draw from
+the squared-exponential CPA prior (4 x 4 rectangles x 4 triangles, nC = 64,
+zero normal boundary velocity, length scale 0.1, variance 1) plus
+N(0, 0.02^2) noise; Gaussian SSD likelihood with sigma = 0.02; random-walk
+proposals N(0, 0.01^2 Sigma_CPA).  One step = one proposal: theta' ->
+Avees -> expm tables -> calc_T_inv of the 65 536 pixel points (t = 1,
+100 steps x 10 RK2 substeps on facet crossings, FP64) -> bilinear remap
+(FP32) -> SSD -> Metropolis accept, all on the GPU; the K timed steps are
+replays of a captured one-step hipGraph.
+
+Prints one JSON line: proposals/s, the live roofline of the dominant kernel
+(calc_T_inv fused with remap + SSD; algorithmic FP64 FLOPs over its
+HIP-event duration), and the CPU oracle timed on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CPAB points integrated/sec + MCMC proposals/sec; serving p50/p99 latency"
+# Algorithmic FLOPs, counted from the reference source (cpab/gpu/calc_2Dto2D.cu):
+# add/sub/mul/div/fmod = 1; compares, min/max, round = 0.
+F_LOOKUP = 15            # compute_cell_idx, type I, inside (:55-56, 62-65, 75-76, 104, 124, 132/140)
+F_AFFINE = 8             # A_times_b_affine (:27-28)
+F_FAST = 2 * F_LOOKUP + F_AFFINE           # 38: one accepted Trel step (:389-391)
+F_SUB = F_LOOKUP + 2 * F_AFFINE + 6 + 4    # 41: one RK2 substep (:179-194)
+F_REMAP_SSD = 16         # bilinear sample (13) + squared error accumulation (3), per pixel
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(problem, n_props, threads):
+    """The CPU oracle on this host: n_props proposals of the same chain step
+    (prep + calc_T_inv on the 65 536 pixel points + remap + SSD)."""
+    import numpy as np
+    from oracle import oracle
+    oracle.lib()
+    sp = problem.space
+    g = sp.tessellation.geom()
+    grid = sp.tessellation.dense_grid_img()
+    p = problem.params
+    rng = np.random.default_rng(1)
+    thetas = [problem.theta_star + 0.01 * rng.standard_normal(len(problem.theta_star)) for _ in range(n_props)]
+    t0 = time.perf_counter()
+    for th in thetas:
+        Av = problem.B @ th
+        sA, T = oracle.prep(2, Av.reshape(sp.nC, -1), -1, p["dt"])
+        inv = oracle.calc_T(g, grid, T, sA, p["dt"], p["nTimeSteps"], p["nStepsODEsolver"], nthreads=threads)
+        w = oracle.remap2d(inv, problem.src, nthreads=threads)
+        oracle.ssd_ll(w, problem.dst, problem.sigma)
+    dt = time.perf_counter() - t0
+    return n_props / dt, dt
+
+
+def config1_extra(torch, ctx):
+    """calc_T_fwd of the 512 x 512 grid (262 144 points, one call incl. prep)."""
+    from cpabdiffeo_amd import configs
+    cfg = configs.config1()
+    sp = cfg["space"]
+    sp.update_pat(cfg["Avees"])
+    pts = torch.from_numpy(cfg["pts"]).cuda()
+    out = torch.zeros_like(pts)
+    nfb = torch.zeros(len(pts), dtype=torch.int32, device="cuda")
+    kw = dict(dt=cfg["dt"], nTimeSteps=100, nStepsODEsolver=10)
+    for _ in range(5):
+        sp._calc_T(pts=pts, mysign=1, out=out, n_fallback=nfb, **kw)
+    torch.cuda.synchronize()
+    n = 50
+    t0 = time.perf_counter()
+    for _ in range(n):
+        sp._calc_T(pts=pts, mysign=1, out=out, **kw)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / n
+    ctx.prof_enable(True)
+    ctx.prof_read()
+    for _ in range(n):
+        sp._calc_T(pts=pts, mysign=1, out=out, n_fallback=nfb, **kw)
+    cnt, ms = ctx.prof_read()
+    ctx.prof_enable(False)
+    kms = ms / cnt
+    fb = int(nfb.sum())
+    flops = len(pts) * 100 * F_FAST + fb * 10 * F_SUB
+    return {"workload": "calc_T_fwd 512x512 grid, nC=64 tess I, 100 steps x 10 substeps, fp64",
+            "points_per_sec": len(pts) / wall, "ms_per_call": wall * 1e3, "kernel_us": kms * 1e3,
+            "fallback_steps": fb, "alg_tflops": flops / (kms * 1e-3) / 1e12,
+            "frac_fp64_peak": flops / (kms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-proposals", type=int, default=24)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build runs on a single MI355X "
+              "(chains and point sets shard across GPUs without communication; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from cpabdiffeo_amd._lib import Context
+    from cpabdiffeo_amd.registration import DeviceMetropolis, RegistrationProblem
+
+    n = 256
+    log("bench: building the 256x256 registration problem")
+    problem = RegistrationProblem(n=n, seed=0)
+    mh = DeviceMetropolis(problem, nChains=1, scale=0.01, seed=0, count_fallbacks=True)
+    mh.set_theta(np.zeros(mh.d))
+    use_graph = not args.no_graph
+    log("bench: warmup %d steps" % args.warmup)
+    mh.run(args.warmup, use_graph=use_graph)
+    torch.cuda.synchronize()
+
+    log("bench: timing %d steps" % args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mh.run(args.steps, use_graph=use_graph)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    proposals_per_sec = args.steps / elapsed
+    nacc = int(mh.n_accepted.cpu().numpy()[0])
+
+    # Roofline of the dominant kernel, live: HIP events around each integration
+    # launch on the stream it runs on, fallback counts from the kernel itself.
+    ctx = Context.get(0)
+    k_steps = min(args.steps, 400)
+    nfb0 = int(mh.n_fallback.cpu().numpy()[0])
+    ctx.prof_enable(True)
+    ctx.prof_read()
+    mh.run(k_steps, use_graph=False)
+    cnt, ms = ctx.prof_read()
+    ctx.prof_enable(False)
+    nfb = int(mh.n_fallback.cpu().numpy()[0]) - nfb0
+    kern_s = ms / cnt * 1e-3
+    npts = n * n
+    flops_per_launch = npts * 100 * F_FAST + (nfb / k_steps) * 10 * F_SUB + npts * F_REMAP_SSD
+    achieved = flops_per_launch / kern_s / 1e12
+
+    extra = None
+    if not args.no_extra:
+        log("bench: config-1 calc_T_fwd extra")
+        extra = config1_extra(torch, ctx)
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            threads = os.cpu_count() or 1
+            omp = os.environ.get("OMP_NUM_THREADS")
+            if omp:
+                threads = int(omp)
+            log("bench: CPU oracle baseline, %d proposals on %d threads" % (args.cpu_proposals, threads))
+            rate, secs = cpu_baseline(problem, args.cpu_proposals, threads)
+            rate1, secs1 = cpu_baseline(problem, 2, 1)
+            cpu = {"value": rate, "unit": "proposals/s", "cores": threads, "kind": "port",
+                   "sample": "%d proposals of this chain step (prep + calc_T_inv 65536 pts + remap + SSD), "
+                             "%.1f s wall on %d threads; single thread: %.3f proposals/s"
+                             % (args.cpu_proposals, secs, threads, rate1),
+                   "single_thread_value": rate1}
+        except Exception as e:  # the oracle is optional on a bare install
+            log("bench: CPU baseline unavailable: %r" % (e,))
+
+    line = {
+        "metric": METRIC,
+        "value": proposals_per_sec,
+        "unit": "proposals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "mcmc-registration 256x256 fp32 blob image, 1 chain, nC=64 tess I, "
+                               "calc_T_inv t=1 100x10 fp64 + bilinear remap + SSD sigma=0.02",
+                   "points_per_proposal": npts, "graph_replay": use_graph,
+                   "accept_ratio": nacc / float(args.warmup + args.steps)},
+        "points_per_sec": proposals_per_sec * npts,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_integrate_ssd (calc_T_inv + remap + SSD)", "kernel_us": kern_s * 1e6,
+                     "alg_flops_per_launch": flops_per_launch, "fallback_steps_per_launch": nfb / k_steps},
+        "cpu_baseline": cpu,
+        "calc_T_fwd_512x512": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

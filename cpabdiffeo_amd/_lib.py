@@ -35,6 +35,11 @@ SIGNATURES = {
     "cpab_remap2d": [_p, _p, _p, _p, _i, _i],
     "cpab_remap3d": [_p, _p, _p, _p, _i, _i, _i],
     "cpab_ssd_ll": [_p, _p, _p, _i64, _d, _p],
+    "cpab_reg2d_create": [_p, _p, ctypes.POINTER(_p)],
+    "cpab_reg2d_init": [_p],
+    "cpab_reg2d_run": [_p, _i, _i],
+    "cpab_reg2d_last_proposal": [_p, _p, _p],
+    "cpab_reg2d_destroy": [_p],
     "cpab_prof_enable": [_p, _i],
     "cpab_prof_read": [_p, ctypes.POINTER(_i64), ctypes.POINTER(_d)],
 }

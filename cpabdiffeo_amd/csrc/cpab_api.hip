@@ -13,6 +13,7 @@
 
 #include "../../include/cpab_hip.h"
 #include "cpab_kernels.h"
+#include "cpab_mcmc.h"
 
 #pragma clang fp contract(off)
 
@@ -495,6 +496,184 @@ int cpab_ssd_ll(cpab_ctx* ctx, const float* x, const float* y, int64_t n, double
   hipLaunchKernelGGL(k_ssd_partial, dim3(nb), dim3(256), 0, ctx->stream, x, y, n, ctx->scratch);
   hipLaunchKernelGGL(k_reduce_ll, dim3(1), dim3(64), 0, ctx->stream, ctx->scratch, nb, sigma, ll);
   HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+// ------------------------------------------------------------- MCMC
+}  // extern "C"
+
+struct cpab_reg2d {
+  cpab_ctx* ctx = nullptr;
+  McmcDesc D{};
+  bool pow2 = true;
+  double* scratch = nullptr;  // theta_prop | ll_prop | tables | partial
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int reg2d_launch_step(cpab_reg2d* r, hipStream_t s, int init) {
+  McmcDesc& D = r->D;
+  const size_t sh_prop = (size_t)(2 * D.d + D.nC * 6) * sizeof(double);
+  hipLaunchKernelGGL(k_mcmc_propose, dim3(D.nChains), dim3(256), sh_prop, s, D, init ? 0 : 1);
+  const size_t sh_int = (size_t)2 * D.nC * 6 * sizeof(double);
+  const dim3 grid(D.nBlocks, D.nChains);
+  // events bracket the integration kernel when profiling (not inside a capture)
+  const bool timed = r->ctx->prof && s == r->ctx->stream;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (timed) {
+    int rc = prof_begin(r->ctx, &a, &b);
+    if (rc) return rc;
+  }
+  if (D.tess == TESS_I) {
+    if (r->pow2) hipLaunchKernelGGL((k_integrate_ssd<TESS_I, true>), grid, dim3(256), sh_int, s, D);
+    else hipLaunchKernelGGL((k_integrate_ssd<TESS_I, false>), grid, dim3(256), sh_int, s, D);
+  } else {
+    if (r->pow2) hipLaunchKernelGGL((k_integrate_ssd<TESS_II, true>), grid, dim3(256), sh_int, s, D);
+    else hipLaunchKernelGGL((k_integrate_ssd<TESS_II, false>), grid, dim3(256), sh_int, s, D);
+  }
+  if (timed) {
+    int rc = prof_end(r->ctx, a, b);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_mcmc_accept, dim3(D.nChains), dim3(64), 0, s, D, init);
+  HIP_OK(r->ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out) {
+  if (!ctx || !in || !out) return CPAB_ERR_ARG;
+  *out = nullptr;
+  const cpab_reg2d_desc& q = *in;
+  Geom g;
+  bool pow2;
+  int rc = make_geom(ctx, 2, q.tess, q.nC0, q.nC1, 0, q.inc_x, q.inc_y, 1.0, &g, &pow2);
+  if (rc) return rc;
+  if ((rc = check_nC(ctx, 2, q.tess, q.nC, q.nC0, q.nC1, 0))) return rc;
+  if (q.nChains < 1 || q.nChains > 65535) return fail(ctx, CPAB_ERR_ARG, "nChains must be in [1, 65535]");
+  if (q.d < 1 || q.d > 8192) return fail(ctx, CPAB_ERR_ARG, "d must be in [1, 8192]");
+  if (q.H < 1 || q.W < 1) return fail(ctx, CPAB_ERR_ARG, "images must be non-empty");
+  if (!(q.sigma > 0.0)) return fail(ctx, CPAB_ERR_ARG, "sigma must be positive");
+  if (q.nTimeSteps < 0 || q.nStepsODE < 0) return fail(ctx, CPAB_ERR_ARG, "negative step counts");
+  if (!q.B || !q.Lprop || !q.src || !q.dst || !q.theta || !q.ll || !q.n_accepted || !q.step)
+    return fail(ctx, CPAB_ERR_ARG, "null pointer in the descriptor");
+  const size_t tb = (size_t)2 * q.nC * 6 * sizeof(double);
+  if (tb > (size_t)ctx->max_lds) return fail(ctx, CPAB_ERR_UNSUPPORTED, "per-cell tables exceed LDS");
+  if ((size_t)(2 * q.d + q.nC * 6) * sizeof(double) > (size_t)ctx->max_lds)
+    return fail(ctx, CPAB_ERR_UNSUPPORTED, "proposal workspace exceeds LDS");
+  cpab_reg2d* r = new (std::nothrow) cpab_reg2d();
+  if (!r) return CPAB_ERR_NOMEM;
+  r->ctx = ctx;
+  r->pow2 = pow2;
+  McmcDesc& D = r->D;
+  D.nChains = q.nChains;
+  D.d = q.d;
+  D.nC = q.nC;
+  D.nC0 = q.nC0;
+  D.nC1 = q.nC1;
+  D.tess = q.tess;
+  D.inc_x = q.inc_x;
+  D.inc_y = q.inc_y;
+  D.H = q.H;
+  D.W = q.W;
+  D.sigma = q.sigma;
+  D.dt = q.dt;
+  D.h = q.dt / (double)q.nStepsODE;
+  D.nT = q.nTimeSteps;
+  D.nS = q.nStepsODE;
+  D.use_ave_ll = q.use_ave_ll;
+  D.inverted = q.inverted_acceptance;
+  D.trace_len = q.trace ? q.trace_len : 0;
+  D.seed_lo = (uint32_t)q.seed;
+  D.seed_hi = (uint32_t)(q.seed >> 32);
+  D.B = q.B;
+  D.L = q.Lprop;
+  D.src = q.src;
+  D.dst = q.dst;
+  D.theta = q.theta;
+  D.ll = q.ll;
+  D.n_acc = q.n_accepted;
+  D.step = q.step;
+  D.trace = q.trace;
+  D.nfb = reinterpret_cast<unsigned long long*>(q.n_fallback);
+  D.g = g;
+  const int64_t nPts = (int64_t)q.H * q.W;
+  D.ptsPerWave = 64;
+  const int64_t waves = (nPts + D.ptsPerWave - 1) / D.ptsPerWave;
+  D.nBlocks = (int)((waves + 3) / 4);
+  const size_t n = (size_t)q.nChains * q.d + q.nChains + (size_t)q.nChains * 2 * q.nC * 6 +
+                   (size_t)q.nChains * D.nBlocks;
+  hipError_t e = hipMalloc(&r->scratch, n * sizeof(double));
+  if (e != hipSuccess) {
+    delete r;
+    return fail(ctx, CPAB_ERR_NOMEM, "hipMalloc: %s", hipGetErrorString(e));
+  }
+  D.theta_prop = r->scratch;
+  D.ll_prop = D.theta_prop + (size_t)q.nChains * q.d;
+  D.tables = D.ll_prop + q.nChains;
+  D.partial = D.tables + (size_t)q.nChains * 2 * q.nC * 6;
+  *out = r;
+  return CPAB_OK;
+}
+
+int cpab_reg2d_init(cpab_reg2d* r) {
+  if (!r) return CPAB_ERR_ARG;
+  return reg2d_launch_step(r, r->ctx->stream, 1);
+}
+
+int cpab_reg2d_run(cpab_reg2d* r, int nSteps, int use_graph) {
+  if (!r) return CPAB_ERR_ARG;
+  cpab_ctx* ctx = r->ctx;
+  if (nSteps < 0) return fail(ctx, CPAB_ERR_ARG, "nSteps must be >= 0");
+  if (!use_graph) {
+    for (int i = 0; i < nSteps; i++) {
+      int rc = reg2d_launch_step(r, ctx->stream, 0);
+      if (rc) return rc;
+    }
+    return CPAB_OK;
+  }
+  if (!r->exec) {
+    // capture one step on a private stream; replays go to the context's stream
+    HIP_OK(ctx, hipStreamCreate(&r->cap));
+    HIP_OK(ctx, hipStreamBeginCapture(r->cap, hipStreamCaptureModeThreadLocal));
+    int rc = reg2d_launch_step(r, r->cap, 0);
+    hipGraph_t gph = nullptr;
+    hipError_t e = hipStreamEndCapture(r->cap, &gph);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(ctx, CPAB_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    r->graph = gph;
+    HIP_OK(ctx, hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
+  }
+  for (int i = 0; i < nSteps; i++) HIP_OK(ctx, hipGraphLaunch(r->exec, ctx->stream));
+  return CPAB_OK;
+}
+
+int cpab_reg2d_last_proposal(cpab_reg2d* r, double* theta_prop, double* ll_prop) {
+  if (!r) return CPAB_ERR_ARG;
+  cpab_ctx* ctx = r->ctx;
+  if (theta_prop)
+    HIP_OK(ctx, hipMemcpyAsync(theta_prop, r->D.theta_prop, sizeof(double) * r->D.nChains * r->D.d,
+                               hipMemcpyDeviceToDevice, ctx->stream));
+  if (ll_prop)
+    HIP_OK(ctx, hipMemcpyAsync(ll_prop, r->D.ll_prop, sizeof(double) * r->D.nChains, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+  return CPAB_OK;
+}
+
+int cpab_reg2d_destroy(cpab_reg2d* r) {
+  if (!r) return CPAB_OK;
+  (void)hipStreamSynchronize(r->ctx->stream);
+  if (r->exec) (void)hipGraphExecDestroy(r->exec);
+  if (r->graph) (void)hipGraphDestroy(r->graph);
+  if (r->cap) (void)hipStreamDestroy(r->cap);
+  if (r->scratch) (void)hipFree(r->scratch);
+  delete r;
   return CPAB_OK;
 }
 

@@ -219,6 +219,10 @@ __device__ __forceinline__ void stage_tables(double* lds, const double* __restri
 //           p = y, c = cell(y); k--; on k == 0 the step is done (t++).
 // solveODE recomputes cell(p) at the top of every substep; here it is the
 // lookup at the end of the previous trip, of the same p: same value.
+//
+// Trips run in an inner loop until some lane finishes its point; only then
+// do lanes retire and refill from the wave's range (the bookkeeping costs
+// ballots and branches, so it stays out of the per-trip path).
 template <int DIM, int TESS, bool POW2, bool LDS, int MODE>
 __global__ void __launch_bounds__(1024)
     k_integrate(const double* __restrict__ pos0, double* __restrict__ out,
@@ -268,40 +272,41 @@ __global__ void __launch_bounds__(1024)
     }
     if (ballot(active) == 0) break;
     if (!active || t >= nT) continue;
-
-    const double* M = (k == 0 ? Ttab : Atab) + c * NE;
-    double y[DIM];
-    affine<DIM>(y, M, p);
-    if (k != 0) {  // RK2 midpoint substep (solveODE, calc_2Dto2D.cu:182-194)
-      double m[DIM];
+    do {  // trips until some lane of the region finishes its point
+      const double* M = (k == 0 ? Ttab : Atab) + c * NE;
+      double y[DIM];
+      affine<DIM>(y, M, p);
+      if (k != 0) {  // RK2 midpoint substep (solveODE, calc_2Dto2D.cu:182-194)
+        double m[DIM];
 #pragma unroll
-      for (int d = 0; d < DIM; d++) m[d] = p[d] + h * y[d] / 2.;
-      affine<DIM>(y, M, m);
+        for (int d = 0; d < DIM; d++) m[d] = p[d] + h * y[d] / 2.;
+        affine<DIM>(y, M, m);
 #pragma unroll
-      for (int d = 0; d < DIM; d++) y[d] = p[d] + y[d] * h;
-    }
-    const int cy = cell_of<DIM, TESS, POW2>(y, g);
-    bool stepped = false;
-    if (k == 0) {
-      if (cy == c) {
+        for (int d = 0; d < DIM; d++) y[d] = p[d] + y[d] * h;
+      }
+      const int cy = cell_of<DIM, TESS, POW2>(y, g);
+      bool stepped = false;
+      if (k == 0) {
+        if (cy == c) {
+#pragma unroll
+          for (int d = 0; d < DIM; d++) p[d] = y[d];
+          stepped = true;
+        } else {
+          nfb++;
+          if (nS > 0) k = nS;
+          else stepped = true;  // zero substeps: the step leaves p where it was
+        }
+      } else {
 #pragma unroll
         for (int d = 0; d < DIM; d++) p[d] = y[d];
-        stepped = true;
-      } else {
-        nfb++;
-        if (nS > 0) k = nS;
-        else stepped = true;  // zero substeps: the step leaves p where it was
+        c = cy;
+        if (--k == 0) stepped = true;
       }
-    } else {
-#pragma unroll
-      for (int d = 0; d < DIM; d++) p[d] = y[d];
-      c = cy;
-      if (--k == 0) stepped = true;
-    }
-    if (stepped) {
-      if (MODE == 1) store_pt<DIM>(out, (int64_t)t * nPts + my, p);
-      t++;
-    }
+      if (stepped) {
+        if (MODE == 1) store_pt<DIM>(out, (int64_t)t * nPts + my, p);
+        t++;
+      }
+    } while (ballot(t >= nT) == 0);
   }
 }
 

@@ -1,0 +1,177 @@
+"""Device-resident MCMC image registration (the config-2 / config-3b path).
+
+Mirrors the reference's registration loop: Metropolis
+(cpab/cpaNd/inference/Metropolis.py:14-151) driving a Gaussian random-walk
+Proposal (cpab/cpaNd/inference/Proposal.py:15-64) and the Gaussian SSD
+likelihood of a warped image
+(cpab/cpa2d/model/transformations/register/_ScaleDependentLogLikelihoodGaussian.py:34-109).
+The reference runs that loop on the host with one blocking device->host copy
+per step; here theta, the log-likelihoods, the counters and the random
+numbers stay on the GPU and many chains advance in the same launches.
+
+Choices, each switchable to the reference's behaviour:
+ * proposal covariance: the full scale^2 * Sigma_CPA (``diagonal=False``) or
+   its diagonal only, as the reference does (Proposal.py:58-64);
+ * acceptance: the Metropolis rule (``inverted_acceptance=False``) or the
+   reference's test log(u) > log_ratio (Metropolis.py:77-82);
+ * ``use_ave_ll`` (default True, as the reference): the log-likelihood is
+   divided by the pixel count (Metropolis.py:31-32, 58-59).
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import configs
+from ._lib import Context, check, lib
+from .gpuarray import require_gpu
+
+
+class RegDesc(ctypes.Structure):
+    """cpab_reg2d_desc of include/cpab_hip.h."""
+    _fields_ = [("nChains", ctypes.c_int), ("d", ctypes.c_int), ("tess", ctypes.c_int), ("nC", ctypes.c_int),
+                ("nC0", ctypes.c_int), ("nC1", ctypes.c_int), ("inc_x", ctypes.c_double),
+                ("inc_y", ctypes.c_double), ("H", ctypes.c_int), ("W", ctypes.c_int),
+                ("sigma", ctypes.c_double), ("dt", ctypes.c_double), ("nTimeSteps", ctypes.c_int),
+                ("nStepsODE", ctypes.c_int), ("use_ave_ll", ctypes.c_int), ("inverted_acceptance", ctypes.c_int),
+                ("seed", ctypes.c_uint64), ("B", ctypes.c_void_p), ("Lprop", ctypes.c_void_p),
+                ("src", ctypes.c_void_p), ("dst", ctypes.c_void_p), ("theta", ctypes.c_void_p),
+                ("ll", ctypes.c_void_p), ("n_accepted", ctypes.c_void_p), ("step", ctypes.c_void_p),
+                ("trace", ctypes.c_void_p), ("trace_len", ctypes.c_int), ("n_fallback", ctypes.c_void_p)]
+
+
+def pixel_basis(space_unit, n):
+    """Basis acting on the pixel domain [0, n]^2: the unit-domain basis with
+    the translation rows scaled by n (same velocity fields, pixel units)."""
+    B = space_unit.B.copy().reshape(space_unit.nC, 2, 3, space_unit.d)
+    B[:, :, 2, :] *= n
+    return B.reshape(space_unit.D, space_unit.d)
+
+
+class RegistrationProblem:
+    """Source/target images, the pixel-domain CPA space and the prior.
+
+    The target is src warped by T^{-1} of theta* (a prior draw) plus
+    N(0, noise^2) pixel noise, computed on the GPU (calc_T_inv + remap_fwd)."""
+
+    def __init__(self, n=256, seed=0, nCs=(4, 4), noise=0.02, sigma=0.02, n_blobs=40,
+                 nTimeSteps=100, nStepsODE=10, t=1.0, length_scale=0.1, variance=1.0):
+        torch = require_gpu()
+        self.n = n
+        self.sigma = sigma
+        self.space_unit = configs.unit_space(nCs, "I")
+        self.space = configs.pixel_space(n, nCs, "I")
+        self.cov = self.space_unit.se_prior_cov(length_scale, variance)
+        self.B = pixel_basis(self.space_unit, n)
+        self.params = dict(dt=t / nTimeSteps, nTimeSteps=nTimeSteps, nStepsODEsolver=nStepsODE)
+        self.src = configs.blob_image(n, n_blobs, seed=seed)
+        rng = np.random.default_rng(seed + 1)
+        self.theta_star = self.space_unit.sample_theta(self.cov, rng)
+        grid = self.space.tessellation.dense_grid_img()
+        self.space.update_pat(self.B @ self.theta_star)
+        pts = torch.from_numpy(grid).cuda()
+        inv = torch.zeros_like(pts)
+        self.space.calc_T_inv(pts=pts, out=inv, **self.params)
+        warped = torch.zeros((n, n), dtype=torch.float32, device="cuda")
+        ctx = Context.get(pts.device.index or 0)
+        ctx.bind_stream(torch.cuda.current_stream().cuda_stream)
+        src_d = torch.from_numpy(self.src).cuda()
+        ctx.call("cpab_remap2d", inv.data_ptr(), src_d.data_ptr(), warped.data_ptr(), n, n)
+        self.dst = (warped.cpu().numpy() + rng.normal(0, noise, (n, n))).astype(np.float32)
+
+
+class DeviceMetropolis:
+    """nChains Metropolis chains of one RegistrationProblem, all on the GPU.
+
+    The API follows Metropolis (set_theta / step / run / record); theta has
+    shape (nChains, d)."""
+
+    def __init__(self, problem, nChains=1, scale=0.01, diagonal=False, use_ave_ll=True,
+                 inverted_acceptance=False, seed=0, trace_len=0, count_fallbacks=False):
+        torch = require_gpu()
+        self.problem = problem
+        self.nChains = nChains
+        sp = problem.space
+        d = problem.space_unit.d
+        self.d = d
+        cov = problem.cov
+        if diagonal:
+            L = np.diag(scale * np.sqrt(np.clip(np.diag(cov), 0, None)))
+        else:
+            C = scale ** 2 * cov
+            L = np.linalg.cholesky(C + 1e-12 * np.trace(C) / d * np.eye(d))
+        self.Lprop = L
+        dev = "cuda"
+        f64 = torch.float64
+        self._B = torch.from_numpy(np.ascontiguousarray(problem.B)).to(dev)
+        self._L = torch.from_numpy(np.ascontiguousarray(L)).to(dev)
+        self._src = torch.from_numpy(problem.src).to(dev)
+        self._dst = torch.from_numpy(problem.dst).to(dev)
+        self.theta = torch.zeros((nChains, d), dtype=f64, device=dev)
+        self.ll = torch.zeros(nChains, dtype=f64, device=dev)
+        self.n_accepted = torch.zeros(nChains, dtype=torch.int64, device=dev)
+        self.step_counter = torch.zeros(nChains, dtype=torch.int64, device=dev)
+        self.trace_len = trace_len
+        self.trace = torch.zeros((nChains, max(trace_len, 1)), dtype=f64, device=dev)
+        self.n_fallback = torch.zeros(nChains, dtype=torch.int64, device=dev)
+        p = problem.params
+        T = sp.tessellation
+        desc = RegDesc(nChains, d, T.type, sp.nC, int(T.nCs[0]), int(T.nCs[1]), float(T.incs[0]),
+                       float(T.incs[1]), problem.n, problem.n, float(problem.sigma), float(p["dt"]),
+                       int(p["nTimeSteps"]), int(p["nStepsODEsolver"]), int(use_ave_ll), int(inverted_acceptance),
+                       int(seed), self._B.data_ptr(), self._L.data_ptr(), self._src.data_ptr(),
+                       self._dst.data_ptr(), self.theta.data_ptr(), self.ll.data_ptr(), self.n_accepted.data_ptr(),
+                       self.step_counter.data_ptr(), self.trace.data_ptr() if trace_len else None, int(trace_len),
+                       self.n_fallback.data_ptr() if count_fallbacks else None)
+        self._ctx = Context.get(0)
+        self._ctx.bind_stream(torch.cuda.current_stream().cuda_stream)
+        h = ctypes.c_void_p()
+        check(lib().cpab_reg2d_create(self._ctx.handle, ctypes.byref(desc), ctypes.byref(h)), self._ctx.handle)
+        self._h = h
+        self.nSteps = 0
+        self.record = {"runs": []}
+
+    def _bind(self):
+        torch = require_gpu()
+        self._ctx.bind_stream(torch.cuda.current_stream().cuda_stream)
+
+    def set_theta(self, theta):
+        """Metropolis.set_theta (Metropolis.py:25-40): set every chain's theta
+        and evaluate its log-likelihood."""
+        torch = require_gpu()
+        th = np.broadcast_to(np.asarray(theta, dtype=np.float64), (self.nChains, self.d))
+        self.theta.copy_(torch.from_numpy(np.ascontiguousarray(th)))
+        self._bind()
+        check(lib().cpab_reg2d_init(self._h), self._ctx.handle)
+
+    def step(self):
+        self.run(1)
+
+    def run(self, N, use_graph=True):
+        """N steps of every chain (Metropolis.run, :104-151), no host sync."""
+        self._bind()
+        t0 = time.perf_counter()
+        check(lib().cpab_reg2d_run(self._h, int(N), 1 if use_graph else 0), self._ctx.handle)
+        self.nSteps += N
+        self.record["runs"].append({"N": N, "enqueue_s": time.perf_counter() - t0})
+
+    def last_proposal(self):
+        torch = require_gpu()
+        tp = torch.zeros((self.nChains, self.d), dtype=torch.float64, device="cuda")
+        lp = torch.zeros(self.nChains, dtype=torch.float64, device="cuda")
+        check(lib().cpab_reg2d_last_proposal(self._h, tp.data_ptr(), lp.data_ptr()), self._ctx.handle)
+        return tp, lp
+
+    def accept_ratio(self):
+        return self.n_accepted.double().cpu().numpy() * 100.0 / max(self.nSteps, 1)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cpab_reg2d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -119,6 +119,62 @@ int cpab_remap3d(cpab_ctx *ctx, const double *pts, const float *vol, float *out,
 int cpab_ssd_ll(cpab_ctx *ctx, const float *x, const float *y, int64_t n, double sigma,
                 double *ll);
 
+/* ------------------------------------------------------------------------
+ * Device-resident Metropolis sampling of 2D image registration, for many
+ * independent chains at once: Metropolis.run / .step
+ * (cpab/cpaNd/inference/Metropolis.py:41-151) with the random-walk proposal
+ * of Proposal.__call__ (cpab/cpaNd/inference/Proposal.py:29-64), theta2Avees
+ * (cpab/cpaNd/_CpaSpace.py:366-376), calc_T_inv and the Gaussian SSD
+ * likelihood (cpab/cpa2d/model/transformations/register/
+ * _ScaleDependentLogLikelihoodGaussian.py:34-109).  The pixel (r, c) of the
+ * H x W images is the point (x = c, y = r) (x_dense_img,
+ * cpab/cpa2d/calcs/_CpaCalcs.py:46-62).  A step is:
+ *   theta' = theta + Lprop z,  z ~ N(0, I) (Philox4x32-10 + Box-Muller,
+ *            counter (step, chain), key = seed);
+ *   Avees  = B theta';  T = calc_T_inv(grid; Avees);
+ *   ll'    = sum -0.5 (src(T(x)) - dst(x))^2 / sigma^2  (/ H*W if use_ave_ll);
+ *   accept if ll' > ll, else if log(u) < ll' - ll (log(u) > ll' - ll with
+ *   inverted_acceptance, the reference's test, Metropolis.py:77-82).
+ * All arrays are device pointers owned by the caller; the object owns only
+ * its scratch. -------------------------------------------------------- */
+typedef struct {
+  int nChains;          /* independent chains sharing the images and basis */
+  int d;                /* dimension of theta */
+  int tess, nC, nC0, nC1;
+  double inc_x, inc_y;
+  int H, W;             /* image size; the domain is [0, W] x [0, H] */
+  double sigma;         /* SSD noise level */
+  double dt;            /* integration: calc_T_inv with (dt, nTimeSteps, nStepsODE) */
+  int nTimeSteps, nStepsODE;
+  int use_ave_ll;       /* divide the log-likelihood by H*W (Metropolis use_ave_ll) */
+  int inverted_acceptance;
+  uint64_t seed;
+  const double *B;      /* (nC*6, d) row-major: Avees = B theta */
+  const double *Lprop;  /* (d, d) row-major lower triangular proposal factor */
+  const float *src;     /* (H, W) source image, warped by the transform */
+  const float *dst;     /* (H, W) target image */
+  double *theta;        /* (nChains, d) current state (in/out) */
+  double *ll;           /* (nChains,) current log-likelihood (out) */
+  int64_t *n_accepted;  /* (nChains,) accepted proposals (in/out) */
+  int64_t *step;        /* (nChains,) step counters (in/out; the RNG counter) */
+  double *trace;        /* optional (nChains, trace_len): ll after each step */
+  int trace_len;
+  uint64_t *n_fallback; /* optional (nChains,): accumulates the integration steps
+                           whose Trel proposal left its cell (RK2 redone) */
+} cpab_reg2d_desc;
+
+typedef struct cpab_reg2d cpab_reg2d;
+
+int cpab_reg2d_create(cpab_ctx *ctx, const cpab_reg2d_desc *desc, cpab_reg2d **out);
+/* ll of the current theta of every chain (Metropolis.set_theta, :25-40). */
+int cpab_reg2d_init(cpab_reg2d *r);
+/* nSteps Metropolis steps of every chain; use_graph != 0 replays a captured
+ * one-step hipGraph instead of launching the three kernels per step. */
+int cpab_reg2d_run(cpab_reg2d *r, int nSteps, int use_graph);
+/* The proposal and its log-likelihood of the last step (nChains x d, nChains). */
+int cpab_reg2d_last_proposal(cpab_reg2d *r, double *theta_prop, double *ll_prop);
+int cpab_reg2d_destroy(cpab_reg2d *r);
+
 /* Kernel timing with HIP events on the context's stream (no reference
  * counterpart; the reference wraps its launches in GpuTimer,
  * cpab/gpu/Calcs.py:537-563).  While enabled, every integration launch

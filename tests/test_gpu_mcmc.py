@@ -1,0 +1,93 @@
+"""Device Metropolis registration vs the CPU reference chain (oracle + numpy
+Philox), graph replay vs launches, and one chain alone vs inside a batch."""
+import sys
+
+import numpy as np
+import pytest
+
+from mcmc_ref import log_likelihood, run_chain
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def problem(gpu):
+    from cpabdiffeo_amd.registration import RegistrationProblem
+    return RegistrationProblem(n=64, seed=3, nTimeSteps=50, nStepsODE=10)
+
+
+def test_target_image_matches_oracle(problem, orc):
+    say("registration target")
+    p = problem
+    Av = p.B @ p.theta_star
+    sA, T = orc.prep(2, Av.reshape(p.space.nC, -1), -1, p.params["dt"])
+    inv = orc.calc_T(p.space.tessellation.geom(), p.space.tessellation.dense_grid_img(), T, sA, p.params["dt"],
+                     p.params["nTimeSteps"], p.params["nStepsODEsolver"])
+    warped = orc.remap2d(inv, p.src)
+    noise = p.dst - warped
+    assert abs(noise.std() - 0.02) < 0.002  # the rest is the N(0, 0.02^2) noise
+
+
+def test_chain_matches_cpu_reference(problem, orc, gpu):
+    from cpabdiffeo_amd.registration import DeviceMetropolis
+    say("chain vs CPU reference, 2 chains x 12 steps")
+    p = problem
+    nsteps = 12
+    mh = DeviceMetropolis(p, nChains=2, scale=0.05, seed=11, trace_len=nsteps)
+    theta0 = np.zeros(mh.d)
+    mh.set_theta(theta0)
+    ll0 = log_likelihood(orc, p.space, p.B, theta0, p.src, p.dst, p.sigma, p.params)
+    assert abs(mh.ll.cpu().numpy()[0] - ll0) <= 1e-9 * abs(ll0)
+    mh.run(nsteps, use_graph=False)
+    trace = mh.trace.cpu().numpy()
+    theta = mh.theta.cpu().numpy()
+    nacc = mh.n_accepted.cpu().numpy()
+    for ch in range(2):
+        ref = run_chain(orc, p.space, p.B, mh.Lprop, theta0, p.src, p.dst, p.sigma, p.params, nsteps, seed=11,
+                        chain=ch)
+        acc = [r[0] for r in ref]
+        assert nacc[ch] == sum(acc), (ch, nacc[ch], acc)
+        np.testing.assert_allclose(trace[ch], [r[2] for r in ref], rtol=1e-9)
+        np.testing.assert_allclose(theta[ch], ref[-1][1], rtol=1e-12, atol=1e-14)
+    assert 0 < nacc.sum() < 2 * nsteps  # both branches of the test ran
+
+
+def test_graph_replay_equals_launches_and_batch_equals_alone(problem, gpu):
+    import torch
+    from cpabdiffeo_amd.registration import DeviceMetropolis
+    say("graph replay and batching")
+    p = problem
+    res = []
+    for nChains, graph in ((1, False), (1, True), (6, True)):
+        mh = DeviceMetropolis(p, nChains=nChains, scale=0.05, seed=5, trace_len=20)
+        mh.set_theta(np.zeros(mh.d))
+        mh.run(20, use_graph=graph)
+        torch.cuda.synchronize()
+        res.append((mh.theta.cpu().numpy(), mh.trace.cpu().numpy(), mh.n_accepted.cpu().numpy()))
+        mh.close()
+    for k in (1, 2):
+        assert np.array_equal(res[k][0][0], res[0][0][0])
+        assert np.array_equal(res[k][1][0], res[0][1][0])
+        assert res[k][2][0] == res[0][2][0]
+    # the other chains of the batch are different chains
+    assert not np.array_equal(res[2][0][1], res[2][0][0])
+
+
+def test_fallback_counter_matches_calc_T(problem, orc, gpu):
+    from cpabdiffeo_amd.registration import DeviceMetropolis
+    say("fallback counter")
+    p = problem
+    mh = DeviceMetropolis(p, nChains=1, scale=0.05, seed=2, count_fallbacks=True)
+    theta = p.theta_star.copy()
+    mh.set_theta(theta)
+    got = int(mh.n_fallback.cpu().numpy()[0])
+    Av = p.B @ theta
+    sA, T = orc.prep(2, Av.reshape(p.space.nC, -1), -1, p.params["dt"])
+    _, nfb, _ = orc.calc_T(p.space.tessellation.geom(), p.space.tessellation.dense_grid_img(), T, sA,
+                           p.params["dt"], p.params["nTimeSteps"], p.params["nStepsODEsolver"], probe=True)
+    assert abs(got - int(nfb.sum())) <= 0.01 * nfb.sum() + 2
