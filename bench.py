@@ -10,12 +10,13 @@ N(0, 0.02^2) noise; Gaussian SSD likelihood with sigma = 0.02; random-walk
 proposals N(0, 0.01^2 Sigma_CPA).  One step = one proposal: theta' ->
 Avees -> expm tables -> calc_T_inv of the 65 536 pixel points (t = 1,
 100 steps x 10 RK2 substeps on facet crossings, FP64) -> bilinear remap
-(FP32) -> SSD -> Metropolis accept, all on the GPU; the K timed steps are
-replays of a captured one-step hipGraph.
+(FP32) -> SSD -> Metropolis accept, all in one kernel launch on the GPU;
+the K timed steps are replays of a captured 16-step hipGraph.
 
 Prints one JSON line: proposals/s, the live roofline of the dominant kernel
-(calc_T_inv fused with remap + SSD; algorithmic FP64 FLOPs over its
-HIP-event duration), and the CPU oracle timed on this host's cores.
+(the one-launch step: proposal, calc_T_inv, remap, SSD, accept; algorithmic
+FP64 FLOPs over its HIP-event duration), and the CPU oracle timed on this
+host's cores.
 """
 import argparse
 import json
@@ -104,7 +105,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-proposals", type=int, default=24)
+    ap.add_argument("--cpu-proposals", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     args = ap.parse_args()
@@ -169,7 +170,7 @@ def main():
                 threads = int(omp)
             log("bench: CPU oracle baseline, %d proposals on %d threads" % (args.cpu_proposals, threads))
             rate, secs = cpu_baseline(problem, args.cpu_proposals, threads)
-            rate1, secs1 = cpu_baseline(problem, 2, 1)
+            rate1, secs1 = cpu_baseline(problem, 4, 1)
             cpu = {"value": rate, "unit": "proposals/s", "cores": threads, "kind": "port",
                    "sample": "%d proposals of this chain step (prep + calc_T_inv 65536 pts + remap + SSD), "
                              "%.1f s wall on %d threads; single thread: %.3f proposals/s"
@@ -198,7 +199,7 @@ def main():
         "points_per_sec": proposals_per_sec * npts,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_integrate_ssd (calc_T_inv + remap + SSD)", "kernel_us": kern_s * 1e6,
+                     "kernel": "k_mcmc_step (proposal + calc_T_inv + remap + SSD + accept, one launch per step)", "kernel_us": kern_s * 1e6,
                      "alg_flops_per_launch": flops_per_launch, "fallback_steps_per_launch": nfb / k_steps},
         "cpu_baseline": cpu,
         "calc_T_fwd_512x512": extra,

@@ -122,6 +122,31 @@ class Calcs:
                  int(p0.shape[0]), *self.parse_nCs_and_incs(dim, incs, nCs))
         return pts_at_T
 
+    def calc_grad_theta(self, xmins, xmaxs, As_vectorized, BasMats, pts0, dt, nTimeSteps, nStepsOdeSolver,
+                        pts_at_T, grad_per_point, dim_domain, dim_range, nCs, incs):
+        """cpab/gpu/Calcs.py:237-335 (kernel calc_2Dto2D.cu:448-491): end points of
+        calc_T_simple into pts_at_T and dT/dtheta into grad_per_point (nPts, dim, d)."""
+        if dim_domain != dim_range:
+            raise ValueError
+        if pts_at_T is None:
+            raise ValueError
+        if len(incs) != dim_domain:
+            raise ValueError(len(incs), dim_domain)
+        dim = dim_domain
+        (A,) = self._tables(dim, As_vectorized)
+        Bs = dev_tensor(BasMats, "float64")
+        d = Bs.shape[0]
+        if tuple(Bs.shape) != (d, self.nCells, dim, dim + 1):
+            raise ValueError(tuple(Bs.shape))
+        p0 = self._pts(pts0, dim)
+        out = self._pts(pts_at_T, dim)
+        grad = dev_tensor(grad_per_point, "float64", (p0.shape[0], dim, d))
+        ctx = context_for(p0)
+        ctx.call("cpab_calc_grad_theta", dim, self.tess_type, self.nCells, p0.data_ptr(), out.data_ptr(),
+                 A.data_ptr(), Bs.data_ptr(), grad.data_ptr(), int(d), float(dt), int(nTimeSteps),
+                 int(nStepsOdeSolver), int(p0.shape[0]), *self.parse_nCs_and_incs(dim, incs, nCs))
+        return grad_per_point
+
     def calc_trajectory(self, xmins, xmaxs, Tlocals_vectorized, As_vectorized, pts0, dt, nTimeSteps,
                         nStepsOdeSolver, dim_domain, dim_range, nCs, incs):
         """cpab/gpu/Calcs.py:735-814.  Returns a device tensor (nTimeSteps, nPts, dim):

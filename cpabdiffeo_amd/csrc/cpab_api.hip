@@ -411,6 +411,51 @@ int cpab_calc_T_simple(cpab_ctx* ctx, int dim, int tess, int nC, const double* p
   return prof_end(ctx, a, b);
 }
 
+int cpab_calc_grad_theta(cpab_ctx* ctx, int dim, int tess, int nC, const double* pos0, double* pos,
+                         const double* As, const double* Bs, double* grad, int d, double dt, int nT, int nS,
+                         int64_t nPts, int nC0, int nC1, int nC2, double ix, double iy, double iz) {
+  if (!ctx) return CPAB_ERR_ARG;
+  Geom g;
+  bool pow2;
+  int r = make_geom(ctx, dim, tess, nC0, nC1, nC2, ix, iy, iz, &g, &pow2);
+  if (r) return r;
+  if ((r = check_nC(ctx, dim, tess, nC, nC0, nC1, nC2))) return r;
+  if (nPts < 0 || nT < 0 || nS < 0 || d < 1) return fail(ctx, CPAB_ERR_ARG, "bad sizes");
+  if (nPts == 0) return CPAB_OK;
+  if (!pos0 || !pos || !As || !Bs || !grad) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  const size_t pb = (size_t)nPts * dim * sizeof(double);
+  if (overlap(pos0, pb, pos, pb)) return fail(ctx, CPAB_ERR_ALIAS, "pts and out cannot point to the same memory");
+  const int64_t steps64 = (int64_t)nS * nT;
+  if (steps64 > 0x7fffffffLL) return fail(ctx, CPAB_ERR_ARG, "nStepsODE*nTimeSteps overflows int");
+  const int ne = dim * (dim + 1);
+  const double h = dt / (double)nS;
+  // columns per thread: 8 when As + 8 basis tables fit in LDS, else 2, else
+  // 2 with the tables read from global memory
+  const size_t lds8 = (size_t)9 * nC * ne * sizeof(double), lds2 = (size_t)3 * nC * ne * sizeof(double);
+  const int jc = lds8 <= (size_t)ctx->max_lds ? 8 : 2;
+  const bool in_lds = jc == 8 || lds2 <= (size_t)ctx->max_lds;
+  const size_t lds = !in_lds ? 0 : (jc == 8 ? lds8 : lds2);
+  const dim3 grid((unsigned)((nPts + 255) / 256), (unsigned)((d + jc - 1) / jc));
+  DISPATCH_GEOM(dim, tess, pow2, {
+    if (jc == 8) {
+      auto k = k_grad_theta<D_, T_, P_, 8, true>;
+      allow_lds(k, lds);
+      hipLaunchKernelGGL(k, grid, dim3(256), lds, ctx->stream, pos0, pos, As, Bs, grad, d, h, (int)steps64, nPts,
+                         g, nC);
+    } else if (in_lds) {
+      auto k = k_grad_theta<D_, T_, P_, 2, true>;
+      allow_lds(k, lds);
+      hipLaunchKernelGGL(k, grid, dim3(256), lds, ctx->stream, pos0, pos, As, Bs, grad, d, h, (int)steps64, nPts,
+                         g, nC);
+    } else {
+      hipLaunchKernelGGL((k_grad_theta<D_, T_, P_, 2, false>), grid, dim3(256), 0, ctx->stream, pos0, pos, As, Bs,
+                         grad, d, h, (int)steps64, nPts, g, nC);
+    }
+  });
+  HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
 int cpab_calc_v(cpab_ctx* ctx, int dim, int tess, int nC, const double* pts, double* vel, const double* As,
                 int64_t nPts, int nC0, int nC1, int nC2, double ix, double iy, double iz) {
   if (!ctx) return CPAB_ERR_ARG;
@@ -502,6 +547,9 @@ int cpab_ssd_ll(cpab_ctx* ctx, const float* x, const float* y, int64_t n, double
 // ------------------------------------------------------------- MCMC
 }  // extern "C"
 
+// Steps per captured graph: graph launches cost several microseconds each.
+constexpr int kGraphSteps = 16;
+
 struct cpab_reg2d {
   cpab_ctx* ctx = nullptr;
   McmcDesc D{};
@@ -516,11 +564,9 @@ namespace {
 
 int reg2d_launch_step(cpab_reg2d* r, hipStream_t s, int init) {
   McmcDesc& D = r->D;
-  const size_t sh_prop = (size_t)(2 * D.d + D.nC * 6) * sizeof(double);
-  hipLaunchKernelGGL(k_mcmc_propose, dim3(D.nChains), dim3(256), sh_prop, s, D, init ? 0 : 1);
-  const size_t sh_int = (size_t)2 * D.nC * 6 * sizeof(double);
+  const size_t sh = (size_t)(2 * D.nC * 6 + D.d + mcmc_prop_doubles(D.nC, D.d)) * sizeof(double);
   const dim3 grid(D.nBlocks, D.nChains);
-  // events bracket the integration kernel when profiling (not inside a capture)
+  // events bracket the step kernel when profiling (not inside a capture)
   const bool timed = r->ctx->prof && s == r->ctx->stream;
   hipEvent_t a = nullptr, b = nullptr;
   if (timed) {
@@ -528,17 +574,16 @@ int reg2d_launch_step(cpab_reg2d* r, hipStream_t s, int init) {
     if (rc) return rc;
   }
   if (D.tess == TESS_I) {
-    if (r->pow2) hipLaunchKernelGGL((k_integrate_ssd<TESS_I, true>), grid, dim3(256), sh_int, s, D);
-    else hipLaunchKernelGGL((k_integrate_ssd<TESS_I, false>), grid, dim3(256), sh_int, s, D);
+    if (r->pow2) hipLaunchKernelGGL((k_mcmc_step<TESS_I, true>), grid, dim3(256), sh, s, D, init);
+    else hipLaunchKernelGGL((k_mcmc_step<TESS_I, false>), grid, dim3(256), sh, s, D, init);
   } else {
-    if (r->pow2) hipLaunchKernelGGL((k_integrate_ssd<TESS_II, true>), grid, dim3(256), sh_int, s, D);
-    else hipLaunchKernelGGL((k_integrate_ssd<TESS_II, false>), grid, dim3(256), sh_int, s, D);
+    if (r->pow2) hipLaunchKernelGGL((k_mcmc_step<TESS_II, true>), grid, dim3(256), sh, s, D, init);
+    else hipLaunchKernelGGL((k_mcmc_step<TESS_II, false>), grid, dim3(256), sh, s, D, init);
   }
   if (timed) {
     int rc = prof_end(r->ctx, a, b);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_mcmc_accept, dim3(D.nChains), dim3(64), 0, s, D, init);
   HIP_OK(r->ctx, hipGetLastError());
   return CPAB_OK;
 }
@@ -563,10 +608,8 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   if (q.nTimeSteps < 0 || q.nStepsODE < 0) return fail(ctx, CPAB_ERR_ARG, "negative step counts");
   if (!q.B || !q.Lprop || !q.src || !q.dst || !q.theta || !q.ll || !q.n_accepted || !q.step)
     return fail(ctx, CPAB_ERR_ARG, "null pointer in the descriptor");
-  const size_t tb = (size_t)2 * q.nC * 6 * sizeof(double);
-  if (tb > (size_t)ctx->max_lds) return fail(ctx, CPAB_ERR_UNSUPPORTED, "per-cell tables exceed LDS");
-  if ((size_t)(2 * q.d + q.nC * 6) * sizeof(double) > (size_t)ctx->max_lds)
-    return fail(ctx, CPAB_ERR_UNSUPPORTED, "proposal workspace exceeds LDS");
+  if ((size_t)(2 * q.nC * 6 + q.d + mcmc_prop_doubles(q.nC, q.d)) * sizeof(double) > (size_t)ctx->max_lds)
+    return fail(ctx, CPAB_ERR_UNSUPPORTED, "per-cell tables and proposal workspace exceed LDS");
   cpab_reg2d* r = new (std::nothrow) cpab_reg2d();
   if (!r) return CPAB_ERR_NOMEM;
   r->ctx = ctx;
@@ -607,10 +650,13 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   D.ptsPerWave = 64;
   const int64_t waves = (nPts + D.ptsPerWave - 1) / D.ptsPerWave;
   D.nBlocks = (int)((waves + 3) / 4);
+  // scratch: theta_prop | ll_prop | tables | partial | partial_nfb | done (as doubles, zeroed once)
   const size_t n = (size_t)q.nChains * q.d + q.nChains + (size_t)q.nChains * 2 * q.nC * 6 +
-                   (size_t)q.nChains * D.nBlocks;
+                   (size_t)2 * q.nChains * D.nBlocks + q.nChains;
   hipError_t e = hipMalloc(&r->scratch, n * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(r->scratch, 0, n * sizeof(double));
   if (e != hipSuccess) {
+    if (r->scratch) (void)hipFree(r->scratch);
     delete r;
     return fail(ctx, CPAB_ERR_NOMEM, "hipMalloc: %s", hipGetErrorString(e));
   }
@@ -618,12 +664,18 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   D.ll_prop = D.theta_prop + (size_t)q.nChains * q.d;
   D.tables = D.ll_prop + q.nChains;
   D.partial = D.tables + (size_t)q.nChains * 2 * q.nC * 6;
+  D.partial_nfb = D.partial + (size_t)q.nChains * D.nBlocks;
+  D.done = reinterpret_cast<unsigned int*>(D.partial_nfb + (size_t)q.nChains * D.nBlocks);
   *out = r;
   return CPAB_OK;
 }
 
 int cpab_reg2d_init(cpab_reg2d* r) {
   if (!r) return CPAB_ERR_ARG;
+  McmcDesc& D = r->D;
+  const size_t sh = (size_t)mcmc_prop_doubles(D.nC, D.d) * sizeof(double);
+  hipLaunchKernelGGL(k_mcmc_prep, dim3(D.nChains), dim3(256), sh, r->ctx->stream, D, 0);
+  HIP_OK(r->ctx, hipGetLastError());
   return reg2d_launch_step(r, r->ctx->stream, 1);
 }
 
@@ -639,10 +691,11 @@ int cpab_reg2d_run(cpab_reg2d* r, int nSteps, int use_graph) {
     return CPAB_OK;
   }
   if (!r->exec) {
-    // capture one step on a private stream; replays go to the context's stream
+    // capture kGraphSteps steps on a private stream; replays go to the context's stream
     HIP_OK(ctx, hipStreamCreate(&r->cap));
     HIP_OK(ctx, hipStreamBeginCapture(r->cap, hipStreamCaptureModeThreadLocal));
-    int rc = reg2d_launch_step(r, r->cap, 0);
+    int rc = CPAB_OK;
+    for (int i = 0; i < kGraphSteps && rc == CPAB_OK; i++) rc = reg2d_launch_step(r, r->cap, 0);
     hipGraph_t gph = nullptr;
     hipError_t e = hipStreamEndCapture(r->cap, &gph);
     if (rc) return rc;
@@ -650,7 +703,12 @@ int cpab_reg2d_run(cpab_reg2d* r, int nSteps, int use_graph) {
     r->graph = gph;
     HIP_OK(ctx, hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
   }
-  for (int i = 0; i < nSteps; i++) HIP_OK(ctx, hipGraphLaunch(r->exec, ctx->stream));
+  const int full = nSteps / kGraphSteps;
+  for (int i = 0; i < full; i++) HIP_OK(ctx, hipGraphLaunch(r->exec, ctx->stream));
+  for (int i = full * kGraphSteps; i < nSteps; i++) {
+    int rc = reg2d_launch_step(r, ctx->stream, 0);
+    if (rc) return rc;
+  }
   return CPAB_OK;
 }
 

@@ -206,19 +206,63 @@ __device__ __forceinline__ void stage_tables(double* lds, const double* __restri
   __syncthreads();
 }
 
+// ----------------------------------------------------------------- trip
+// One unit of the reference's step schedule for one point (see k_integrate):
+//   k == 0: y = Trel_c [p;1]; if cell(y) == c accept, else start the RK2 redo
+//           (k = nStepsODE; p and c unchanged)   (calc_2Dto2D.cu:389-401)
+//   k >  0: one RK2 midpoint substep of A_c from p (solveODE, :179-194);
+//           p = y, c = cell(y), k--.
+// Both kinds of trip share the table read, the first affine product and the
+// cell lookup, so lanes in different modes diverge only over the midpoint
+// arithmetic.  (Measured on gfx950: evaluating the midpoint in every lane
+// and selecting, or chaining several Trel proposals speculatively so their
+// lookups overlap, are both slower than this.)
+// Returns true when the step is complete (the caller advances t).
+template <int DIM, int TESS, bool POW2>
+__device__ __forceinline__ bool trip(double* p, int& c, int& k, int& nfb, const double* Ttab,
+                                     const double* Atab, double h, int nS, const Geom& g) {
+  constexpr int NE = DIM * (DIM + 1);
+  const bool fast = (k == 0);
+  const double* M = (fast ? Ttab : Atab) + c * NE;
+  double y[DIM];
+  affine<DIM>(y, M, p);
+  if (!fast) {  // RK2 midpoint substep (solveODE, calc_2Dto2D.cu:182-194)
+    double m[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) m[d] = p[d] + h * y[d] / 2.;
+    affine<DIM>(y, M, m);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) y[d] = p[d] + y[d] * h;
+  }
+  const int cy = cell_of<DIM, TESS, POW2>(y, g);
+  if (!fast) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) p[d] = y[d];
+    c = cy;
+    return --k == 0;
+  }
+  if (cy == c) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) p[d] = y[d];
+    return true;
+  }
+  nfb++;
+  if (nS > 0) {
+    k = nS;
+    return false;
+  }
+  return true;  // zero substeps: the step leaves p where it was
+}
+
 // --------------------------------------------------------- integration
 // MODE 0: calc_T (write the end point).  MODE 1: calc_trajectory (write the
 // point after every step into traj[(t*nPts + i)*DIM]).
 //
 // Lane state: p (current point), c (cell of p), t (steps done),
 // k (RK2 substeps still to run for the current step; 0 = next trip is a
-// Trel proposal).  One trip:
-//   k == 0: y = Trel_c [p;1]; if cell(y) == c accept (t++, c unchanged) else
-//           reject (k = nStepsODE, p and c unchanged)          (:389-401)
-//   k >  0: y = RK2 substep of A_c from p (solveODE's body, :179-194);
-//           p = y, c = cell(y); k--; on k == 0 the step is done (t++).
-// solveODE recomputes cell(p) at the top of every substep; here it is the
-// lookup at the end of the previous trip, of the same p: same value.
+// Trel proposal); one loop trip is one `trip`.  solveODE recomputes cell(p)
+// at the top of every substep; here it is the lookup at the end of the
+// previous trip, of the same p: same value.
 //
 // Trips run in an inner loop until some lane finishes its point; only then
 // do lanes retire and refill from the wave's range (the bookkeeping costs
@@ -273,41 +317,88 @@ __global__ void __launch_bounds__(1024)
     if (ballot(active) == 0) break;
     if (!active || t >= nT) continue;
     do {  // trips until some lane of the region finishes its point
-      const double* M = (k == 0 ? Ttab : Atab) + c * NE;
-      double y[DIM];
-      affine<DIM>(y, M, p);
-      if (k != 0) {  // RK2 midpoint substep (solveODE, calc_2Dto2D.cu:182-194)
-        double m[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; d++) m[d] = p[d] + h * y[d] / 2.;
-        affine<DIM>(y, M, m);
-#pragma unroll
-        for (int d = 0; d < DIM; d++) y[d] = p[d] + y[d] * h;
-      }
-      const int cy = cell_of<DIM, TESS, POW2>(y, g);
-      bool stepped = false;
-      if (k == 0) {
-        if (cy == c) {
-#pragma unroll
-          for (int d = 0; d < DIM; d++) p[d] = y[d];
-          stepped = true;
-        } else {
-          nfb++;
-          if (nS > 0) k = nS;
-          else stepped = true;  // zero substeps: the step leaves p where it was
-        }
-      } else {
-#pragma unroll
-        for (int d = 0; d < DIM; d++) p[d] = y[d];
-        c = cy;
-        if (--k == 0) stepped = true;
-      }
-      if (stepped) {
+      if (trip<DIM, TESS, POW2>(p, c, k, nfb, Ttab, Atab, h, nS, g)) {
         if (MODE == 1) store_pt<DIM>(out, (int64_t)t * nPts + my, p);
         t++;
       }
     } while (ballot(t >= nT) == 0);
   }
+}
+
+// ---------------------------------------------------------- sensitivity
+// calc_grad_theta (cpab/gpu/calc_2Dto2D.cu:202-319, 448-491): calc_T_simple
+// together with the forward sensitivity q_j = dp/dtheta_j of every basis
+// column j, integrated by the same RK2 (q' = B_j [p;1] + A_lin q).  Grid
+// (point blocks, column chunks): a thread owns one point and JC columns,
+// whose basis tables (layout (d, nC, DIM, DIM+1)) sit in LDS with As; the
+// reference keeps q in global memory and re-reads it every substep, here it
+// stays in registers (same values).  Output grad (nPts, DIM, d) and, from
+// the first chunk, the end points.  JC = columns per thread (the widest
+// whose tables fit in LDS); LDS = false reads the tables from global memory.
+template <int DIM, int TESS, bool POW2, int JC, bool LDS>
+__global__ void __launch_bounds__(256)
+    k_grad_theta(const double* __restrict__ pos0, double* __restrict__ pos, const double* __restrict__ As,
+                 const double* __restrict__ Bs, double* __restrict__ grad, int d, double h, int nSteps,
+                 int64_t nPts, Geom g, int nC) {
+  constexpr int NE = DIM * (DIM + 1);
+  extern __shared__ double lds[];
+  const int j0 = blockIdx.y * JC;
+  const int nj = d - j0 < JC ? d - j0 : JC;
+  const double* Al = As;
+  const double* Bl = Bs + (int64_t)j0 * nC * NE;
+  if (LDS) {
+    double* a = lds;
+    double* b = lds + nC * NE;
+    for (int i = threadIdx.x; i < nC * NE; i += blockDim.x) a[i] = As[i];
+    for (int i = threadIdx.x; i < nj * nC * NE; i += blockDim.x) b[i] = Bs[(int64_t)j0 * nC * NE + i];
+    __syncthreads();
+    Al = a;
+    Bl = b;
+  }
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nPts) return;
+  double p[DIM], v[DIM], pMid[DIM], vMid[DIM];
+  double q[JC][DIM];
+#pragma unroll
+  for (int jj = 0; jj < JC; jj++)
+#pragma unroll
+    for (int i = 0; i < DIM; i++) q[jj][i] = 0.0;
+  load_pt<DIM>(pos0, idx, p);
+  for (int s = 0; s < nSteps; s++) {
+    const int c = cell_of<DIM, TESS, POW2>(p, g);
+    const double* A = Al + c * NE;
+    affine<DIM>(v, A, p);
+#pragma unroll
+    for (int i = 0; i < DIM; i++) pMid[i] = p[i] + h * v[i] / 2.;
+    affine<DIM>(vMid, A, pMid);
+#pragma unroll
+    for (int jj = 0; jj < JC; jj++) {
+      if (jj < nj) {
+        const double* Bj = Bl + (jj * nC + c) * NE;
+        double BtT[DIM], AdT[DIM], u[DIM], qMid[DIM];
+        affine<DIM>(BtT, Bj, p);
+        linear<DIM>(AdT, A, q[jj]);
+#pragma unroll
+        for (int i = 0; i < DIM; i++) u[i] = BtT[i] + AdT[i];
+#pragma unroll
+        for (int i = 0; i < DIM; i++) qMid[i] = q[jj][i] + h * u[i] / 2.;
+        affine<DIM>(BtT, Bj, pMid);
+        linear<DIM>(AdT, A, qMid);
+#pragma unroll
+        for (int i = 0; i < DIM; i++) u[i] = BtT[i] + AdT[i];
+#pragma unroll
+        for (int i = 0; i < DIM; i++) q[jj][i] += u[i] * h;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < DIM; i++) p[i] += vMid[i] * h;
+  }
+  if (blockIdx.y == 0) store_pt<DIM>(pos, idx, p);
+#pragma unroll
+  for (int jj = 0; jj < JC; jj++)
+    if (jj < nj)
+#pragma unroll
+      for (int i = 0; i < DIM; i++) grad[(idx * DIM + i) * d + j0 + jj] = q[jj][i];
 }
 
 // calc_T_simple: nStepsODE * nTimeSteps RK2 substeps (calc_2Dto2D.cu:412-445).

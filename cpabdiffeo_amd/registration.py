@@ -156,6 +156,7 @@ class DeviceMetropolis:
         self.record["runs"].append({"N": N, "enqueue_s": time.perf_counter() - t0})
 
     def last_proposal(self):
+        """(theta' the next step will evaluate, ll of the last evaluated proposal)."""
         torch = require_gpu()
         tp = torch.zeros((self.nChains, self.d), dtype=torch.float64, device="cuda")
         lp = torch.zeros(self.nChains, dtype=torch.float64, device="cuda")

@@ -420,6 +420,28 @@ class CpaSpace:
                                       out, self.dim_domain, self.dim_range, self.nCs, self.incs, timer=timer)
         return out
 
+    def calc_grad_theta(self, pat=None, pts=None, mysign=1, do_checks=False, transformed=None,
+                        grad_theta=None, grad_per_point=None, **params_flow_int):
+        """cpab/cpaNd/_CpaSpace.py:669-686: calc_T_simple end points into
+        `transformed` and dT/dtheta (nPts, dim, d) into `grad_per_point`.
+        With mysign = -1 the field is -v but the basis tables are used as
+        they are, as in the reference's kernel call (_CpaCalcs.py:770-792)."""
+        if pts is None or transformed is None or grad_per_point is None:
+            raise ValueError
+        self._check_pts(pts, transformed)
+        from .calcs import gpu_prep_trels
+        from .gpuarray import CpuGpuArray
+        pat = self.pat if pat is None else pat
+        g = self._dev()
+        gpu_prep_trels(self.dim_domain, pat.device_table(), int(mysign), 0.0, g["signed_As"], g["Tlocals"])
+        if "BasMats" not in g:
+            g["BasMats"] = CpuGpuArray(self.BasMats)
+        self._gpu_calcs.calc_grad_theta(None, None, g["signed_As"], g["BasMats"], pts, params_flow_int["dt"],
+                                        params_flow_int["nTimeSteps"], params_flow_int["nStepsODEsolver"],
+                                        transformed, grad_per_point, self.dim_domain, self.dim_range,
+                                        self.nCs, self.incs)
+        return grad_per_point
+
     def calc_v(self, pat=None, pts=None, out=None, do_checks=True):
         """cpab/cpaNd/_CpaSpace.py:573-587 -> _CpaCalcs.py:95-165."""
         if pts is None:

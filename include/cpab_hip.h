@@ -92,6 +92,17 @@ int cpab_calc_trajectory(cpab_ctx *ctx, int dim, int tess, int nC, const double 
                          int nTimeSteps, int nStepsODE, int64_t nPts, int nC0, int nC1, int nC2,
                          double inc_x, double inc_y, double inc_z);
 
+/* calc_grad_theta: the forward sensitivity of calc_T_simple
+ * (cpab/gpu/calc_2Dto2D.cu:202-319, 448-491; bound nowhere in the reference,
+ * cpab/gpu/Calcs.py:196, 237-335).  Bs: (d, nC, dim, dim+1) basis tables
+ * (CpaSpace.BasMats, cpab/cpaNd/_CpaSpace.py:296-300); grad: (nPts, dim, d),
+ * dT(x)/dtheta with the same RK2 (q_j' = B_j [p;1] + A_lin q_j); pos: the
+ * calc_T_simple end points. */
+int cpab_calc_grad_theta(cpab_ctx *ctx, int dim, int tess, int nC, const double *pos0, double *pos,
+                         const double *As, const double *Bs, double *grad, int d, double dt,
+                         int nTimeSteps, int nStepsODE, int64_t nPts, int nC0, int nC1, int nC2,
+                         double inc_x, double inc_y, double inc_z);
+
 /* calc_v: Calcs.calc_velocities (cpab/gpu/Calcs.py:816-924), kernel
  * cpab/gpu/calc_2Dto2D.cu:561-602.  As is the unsigned table. */
 int cpab_calc_v(cpab_ctx *ctx, int dim, int tess, int nC, const double *pts, double *vel,
@@ -168,10 +179,13 @@ typedef struct cpab_reg2d cpab_reg2d;
 int cpab_reg2d_create(cpab_ctx *ctx, const cpab_reg2d_desc *desc, cpab_reg2d **out);
 /* ll of the current theta of every chain (Metropolis.set_theta, :25-40). */
 int cpab_reg2d_init(cpab_reg2d *r);
-/* nSteps Metropolis steps of every chain; use_graph != 0 replays a captured
- * one-step hipGraph instead of launching the three kernels per step. */
+/* nSteps Metropolis steps of every chain (one kernel launch per step);
+ * use_graph != 0 replays a captured 16-step hipGraph instead of launching
+ * (the remainder of nSteps/16 is launched directly). */
 int cpab_reg2d_run(cpab_reg2d *r, int nSteps, int use_graph);
-/* The proposal and its log-likelihood of the last step (nChains x d, nChains). */
+/* theta_prop (nChains x d): the proposal the next step will evaluate (armed
+ * at the end of every step); ll_prop (nChains): the log-likelihood of the
+ * proposal evaluated by the last step.  Either pointer may be NULL. */
 int cpab_reg2d_last_proposal(cpab_reg2d *r, double *theta_prop, double *ll_prop);
 int cpab_reg2d_destroy(cpab_reg2d *r);
 

@@ -302,3 +302,25 @@ def test_errors_and_edge_sizes(gpu):
     assert torch.equal(out, pts)  # zero steps: a copy
     c.calc_T(None, None, tab, tab, pts, 0.01, 10, 0, out, 2, 2, sp.nCs, sp.incs)  # zero substeps terminates
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("name,lo,hi,nCs,tess", [GEOMS[0], GEOMS[2], GEOMS[6], GEOMS[8]])
+def test_grad_theta_bitexact(gpu, orc, name, lo, hi, nCs, tess):
+    """Forward sensitivity (calc_2Dto2D.cu:202-319) vs the oracle, bit for bit."""
+    torch = gpu
+    say("grad_theta", name)
+    sp = space_for(lo, hi, nCs, tess)
+    rng = np.random.default_rng(27)
+    theta = rng.normal(size=sp.d) * 0.3
+    Av = sp.B @ theta
+    pts = rng.uniform(0.05, 0.95, (3000, sp.dim_domain)) * sp.XMAXS
+    sA, _ = orc.prep(sp.dim_domain, Av.reshape(sp.nC, -1), 1, 0.05)
+    wpos, wgrad = orc.calc_grad_theta(sp.tessellation.geom(), pts, sA, sp.BasMats, 0.05, 20, 5)
+    sp.update_pat(Av)
+    out = torch.zeros(pts.shape, dtype=torch.float64, device="cuda")
+    grad = torch.zeros((len(pts), sp.dim_domain, sp.d), dtype=torch.float64, device="cuda")
+    sp.calc_grad_theta(pts=dev(torch, pts), transformed=out, grad_per_point=grad, dt=0.05, nTimeSteps=20,
+                       nStepsODEsolver=5)
+    assert np.array_equal(host(out), wpos)
+    g = host(grad)
+    assert np.array_equal(g, wgrad), np.abs(g - wgrad).max()
