@@ -10,13 +10,18 @@ N(0, 0.02^2) noise; Gaussian SSD likelihood with sigma = 0.02; random-walk
 proposals N(0, 0.01^2 Sigma_CPA).  One step = one proposal: theta' ->
 Avees -> expm tables -> calc_T_inv of the 65 536 pixel points (t = 1,
 100 steps x 10 RK2 substeps on facet crossings, FP64) -> bilinear remap
-(FP32) -> SSD -> Metropolis accept, all in one kernel launch on the GPU;
-the K timed steps are replays of a captured 16-step hipGraph.
+(FP32) -> SSD -> Metropolis accept, on the GPU, launched from captured
+hipGraphs.  After W warmup steps the timed region is the K-step chain from
+theta = 0 with its counters reset, so the timed work does not depend on W.
+
+Each launch evaluates the next 16 proposals on the all-accepted path and
+decides them in order (speculative prefetch): the chain is the same bit
+for bit as one step per launch, which is timed too and reported beside it.
 
 Prints one JSON line: proposals/s, the live roofline of the dominant kernel
-(the one-launch step: proposal, calc_T_inv, remap, SSD, accept; algorithmic
-FP64 FLOPs over its HIP-event duration), and the CPU oracle timed on this
-host's cores.
+(algorithmic FP64 FLOPs of the proposals each launch evaluates, over the
+launch's HIP-event duration inside the timed region), and the CPU oracle
+timed on this host's cores.
 """
 import argparse
 import json
@@ -124,36 +129,53 @@ def main():
     n = 256
     log("bench: building the 256x256 registration problem")
     problem = RegistrationProblem(n=n, seed=0)
-    mh = DeviceMetropolis(problem, nChains=1, scale=0.01, seed=0, count_fallbacks=True)
-    mh.set_theta(np.zeros(mh.d))
     use_graph = not args.no_graph
-    log("bench: warmup %d steps" % args.warmup)
-    mh.run(args.warmup, use_graph=use_graph)
-    torch.cuda.synchronize()
-
-    log("bench: timing %d steps" % args.steps)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    mh.run(args.steps, use_graph=use_graph)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    proposals_per_sec = args.steps / elapsed
-    nacc = int(mh.n_accepted.cpu().numpy()[0])
-
-    # Roofline of the dominant kernel, live: HIP events around each integration
-    # launch on the stream it runs on, fallback counts from the kernel itself.
     ctx = Context.get(0)
-    k_steps = min(args.steps, 400)
-    nfb0 = int(mh.n_fallback.cpu().numpy()[0])
-    ctx.prof_enable(True)
-    ctx.prof_read()
-    mh.run(k_steps, use_graph=False)
-    cnt, ms = ctx.prof_read()
-    ctx.prof_enable(False)
-    nfb = int(mh.n_fallback.cpu().numpy()[0]) - nfb0
+
+    def chain(spec_depth, warmup, steps, timed_events):
+        """warmup steps, then the timed chain: theta = 0, counters reset, `steps` steps."""
+        mh = DeviceMetropolis(problem, nChains=1, scale=0.01, seed=0, count_fallbacks=True, spec_depth=spec_depth)
+        mh.set_theta(np.zeros(mh.d))
+        mh.run(warmup, use_graph=use_graph)
+        torch.cuda.synchronize()
+        for t in (mh.step_counter, mh.n_accepted, mh.n_fallback, mh.stats):
+            t.zero_()
+        mh.set_theta(np.zeros(mh.d))
+        torch.cuda.synchronize()
+        if timed_events:
+            ctx.prof_enable(True)
+            ctx.prof_read()
+        t0 = time.perf_counter()
+        mh.run(steps, use_graph=use_graph)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        ev = ctx.prof_read() if timed_events else None
+        ctx.prof_enable(False)
+        out = dict(elapsed=elapsed, events=ev, stats=mh.stats.cpu().numpy()[0].copy(),
+                   nacc=int(mh.n_accepted.cpu().numpy()[0]), nfb=int(mh.n_fallback.cpu().numpy()[0]),
+                   ll=float(mh.ll.cpu().numpy()[0]), theta=mh.theta.cpu().numpy()[0].copy(), K=mh.spec_depth)
+        mh.close()
+        return out
+
+    log("bench: warmup %d steps, then the timed %d-step chain (speculative prefetch)" % (args.warmup, args.steps))
+    spec = chain(0, args.warmup, args.steps, True)
+    log("bench: the same chain one step per launch")
+    seq = chain(1, min(args.warmup, 200), args.steps, False)
+    same_chain = bool(np.array_equal(spec["theta"], seq["theta"]) and spec["ll"] == seq["ll"]
+                      and spec["nacc"] == seq["nacc"] and spec["nfb"] == seq["nfb"])
+    elapsed = spec["elapsed"]
+    proposals_per_sec = args.steps / elapsed
+    nacc = spec["nacc"]
+
+    # Roofline of the dominant kernel, live over the timed region: HIP events
+    # on the launch stream around every launch / graph replay, and the
+    # kernel's own counts of evaluated proposals and their fallback steps.
+    cnt, ms = spec["events"]
+    st = spec["stats"]  # proposals evaluated, their fallbacks, launches, steps decided
+    launches = int(st[2])
     kern_s = ms / cnt * 1e-3
     npts = n * n
-    flops_per_launch = npts * 100 * F_FAST + (nfb / k_steps) * 10 * F_SUB + npts * F_REMAP_SSD
+    flops_per_launch = (st[0] * npts * (100 * F_FAST + F_REMAP_SSD) + st[1] * 10 * F_SUB) / launches
     achieved = flops_per_launch / kern_s / 1e12
 
     extra = None
@@ -192,15 +214,23 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "mcmc-registration 256x256 fp32 blob image, 1 chain, nC=64 tess I, "
-                               "calc_T_inv t=1 100x10 fp64 + bilinear remap + SSD sigma=0.02",
-                   "points_per_proposal": npts, "graph_replay": use_graph,
-                   "accept_ratio": nacc / float(args.warmup + args.steps)},
+        "config": {"workload": "mcmc-registration 256x256 fp32 blob image, 1 chain from theta=0, %d steps, "
+                               "nC=64 tess I, calc_T_inv t=1 100x10 fp64 + bilinear remap + SSD sigma=0.02"
+                               % args.steps,
+                   "points_per_proposal": npts, "graph_replay": use_graph, "spec_depth": spec["K"],
+                   "accept_ratio": nacc / float(args.steps)},
         "points_per_sec": proposals_per_sec * npts,
+        "one_step_per_launch": {"proposals_per_sec": args.steps / seq["elapsed"],
+                                "speedup_of_speculation": seq["elapsed"] / elapsed,
+                                "same_chain_bit_for_bit": same_chain},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_mcmc_step (proposal + calc_T_inv + remap + SSD + accept, one launch per step)", "kernel_us": kern_s * 1e6,
-                     "alg_flops_per_launch": flops_per_launch, "fallback_steps_per_launch": nfb / k_steps},
+                     "kernel": "k_mcmc_step (K proposals: calc_T_inv + remap + SSD; accept; arm next)",
+                     "kernel_us": kern_s * 1e6, "launches": launches,
+                     "proposals_evaluated_per_launch": st[0] / launches,
+                     "steps_decided_per_launch": st[3] / launches,
+                     "alg_flops_per_launch": flops_per_launch,
+                     "fallback_steps_per_launch": st[1] / launches},
         "cpu_baseline": cpu,
         "calc_T_fwd_512x512": extra,
     }

@@ -27,6 +27,7 @@ struct cpab_ctx {
   int n_cu = 256;
   bool prof = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;  // pending timed launches
+  std::vector<int> ev_n;                               // kernel launches each pair brackets
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free;
   int64_t prof_n = 0;
   double prof_ms = 0.0;
@@ -164,10 +165,11 @@ int prof_begin(cpab_ctx* ctx, hipEvent_t* a, hipEvent_t* b) {
   return CPAB_OK;
 }
 
-int prof_end(cpab_ctx* ctx, hipEvent_t a, hipEvent_t b) {
+int prof_end(cpab_ctx* ctx, hipEvent_t a, hipEvent_t b, int launches = 1) {
   if (!ctx->prof) return CPAB_OK;
   HIP_OK(ctx, hipEventRecord(b, ctx->stream));
   ctx->ev.push_back({a, b});
+  ctx->ev_n.push_back(launches);
   return CPAB_OK;
 }
 
@@ -547,25 +549,35 @@ int cpab_ssd_ll(cpab_ctx* ctx, const float* x, const float* y, int64_t n, double
 // ------------------------------------------------------------- MCMC
 }  // extern "C"
 
-// Steps per captured graph: graph launches cost several microseconds each.
+// Launches per captured graph: graph launches cost several microseconds each.
 constexpr int kGraphSteps = 16;
+// Speculation depth chosen when the descriptor leaves it to the library:
+// enough slots that one launch integrates about 2^20 points.
+constexpr int64_t kSpecPoints = 1 << 20;
+constexpr int kMaxSpec = 32;
 
 struct cpab_reg2d {
   cpab_ctx* ctx = nullptr;
   McmcDesc D{};
   bool pow2 = true;
-  double* scratch = nullptr;  // theta_prop | ll_prop | tables | partial
+  double* scratch = nullptr;  // theta_prop | ll_prop | tables | partials | stop | done
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<int64_t> host_step;
 };
 
 namespace {
 
+size_t reg2d_lds(const McmcDesc& D) {
+  const int a = 2 * D.nC * 6, b = mcmc_finish_doubles(D.K, D.d);
+  return (size_t)(a > b ? a : b) * sizeof(double);
+}
+
 int reg2d_launch_step(cpab_reg2d* r, hipStream_t s, int init) {
   McmcDesc& D = r->D;
-  const size_t sh = (size_t)(2 * D.nC * 6 + D.d + mcmc_prop_doubles(D.nC, D.d)) * sizeof(double);
-  const dim3 grid(D.nBlocks, D.nChains);
+  const size_t sh = reg2d_lds(D);
+  const dim3 grid((unsigned)(D.nBlocks * D.K), D.nChains);
   // events bracket the step kernel when profiling (not inside a capture)
   const bool timed = r->ctx->prof && s == r->ctx->stream;
   hipEvent_t a = nullptr, b = nullptr;
@@ -585,6 +597,45 @@ int reg2d_launch_step(cpab_reg2d* r, hipStream_t s, int init) {
     if (rc) return rc;
   }
   HIP_OK(r->ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+// n launches of the step kernel: replays of the captured kGraphSteps-launch
+// graph, then the remainder directly.
+int reg2d_launches(cpab_reg2d* r, int64_t n, int use_graph) {
+  cpab_ctx* ctx = r->ctx;
+  if (!use_graph) {
+    for (int64_t i = 0; i < n; i++) {
+      int rc = reg2d_launch_step(r, ctx->stream, 0);
+      if (rc) return rc;
+    }
+    return CPAB_OK;
+  }
+  if (!r->exec) {
+    // capture kGraphSteps launches on a private stream; replays go to the context's stream
+    HIP_OK(ctx, hipStreamCreate(&r->cap));
+    HIP_OK(ctx, hipStreamBeginCapture(r->cap, hipStreamCaptureModeThreadLocal));
+    int rc = CPAB_OK;
+    for (int i = 0; i < kGraphSteps && rc == CPAB_OK; i++) rc = reg2d_launch_step(r, r->cap, 0);
+    hipGraph_t gph = nullptr;
+    hipError_t e = hipStreamEndCapture(r->cap, &gph);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(ctx, CPAB_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    r->graph = gph;
+    HIP_OK(ctx, hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
+  }
+  const int64_t full = n / kGraphSteps;
+  for (int64_t i = 0; i < full; i++) {
+    hipEvent_t a = nullptr, b = nullptr;
+    int rc = prof_begin(ctx, &a, &b);
+    if (rc) return rc;
+    HIP_OK(ctx, hipGraphLaunch(r->exec, ctx->stream));
+    if ((rc = prof_end(ctx, a, b, kGraphSteps))) return rc;
+  }
+  for (int64_t i = full * kGraphSteps; i < n; i++) {
+    int rc = reg2d_launch_step(r, ctx->stream, 0);
+    if (rc) return rc;
+  }
   return CPAB_OK;
 }
 
@@ -608,8 +659,19 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   if (q.nTimeSteps < 0 || q.nStepsODE < 0) return fail(ctx, CPAB_ERR_ARG, "negative step counts");
   if (!q.B || !q.Lprop || !q.src || !q.dst || !q.theta || !q.ll || !q.n_accepted || !q.step)
     return fail(ctx, CPAB_ERR_ARG, "null pointer in the descriptor");
-  if ((size_t)(2 * q.nC * 6 + q.d + mcmc_prop_doubles(q.nC, q.d)) * sizeof(double) > (size_t)ctx->max_lds)
-    return fail(ctx, CPAB_ERR_UNSUPPORTED, "per-cell tables and proposal workspace exceed LDS");
+  if (q.spec_depth < 0 || q.spec_depth > kMaxSpec)
+    return fail(ctx, CPAB_ERR_ARG, "spec_depth must be in [0, %d]", kMaxSpec);
+  const int64_t nPts = (int64_t)q.H * q.W;
+  int K = q.spec_depth;
+  if (K == 0) {
+    const int64_t k = kSpecPoints / (nPts * (int64_t)q.nChains);
+    K = k < 1 ? 1 : (k > 16 ? 16 : (int)k);
+  }
+  // the epilogue's workspace must fit in LDS beside nothing else
+  while (K > 1 && (size_t)mcmc_finish_doubles(K, q.d) * sizeof(double) > (size_t)ctx->max_lds) K--;
+  if ((size_t)(2 * q.nC * 6) * sizeof(double) > (size_t)ctx->max_lds ||
+      (size_t)mcmc_finish_doubles(K, q.d) * sizeof(double) > (size_t)ctx->max_lds)
+    return fail(ctx, CPAB_ERR_UNSUPPORTED, "per-cell tables or proposal workspace exceed LDS");
   cpab_reg2d* r = new (std::nothrow) cpab_reg2d();
   if (!r) return CPAB_ERR_NOMEM;
   r->ctx = ctx;
@@ -645,14 +707,19 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   D.step = q.step;
   D.trace = q.trace;
   D.nfb = reinterpret_cast<unsigned long long*>(q.n_fallback);
+  D.stats = q.stats;
+  D.K = K;
   D.g = g;
-  const int64_t nPts = (int64_t)q.H * q.W;
   D.ptsPerWave = 64;
   const int64_t waves = (nPts + D.ptsPerWave - 1) / D.ptsPerWave;
   D.nBlocks = (int)((waves + 3) / 4);
-  // scratch: theta_prop | ll_prop | tables | partial | partial_nfb | done (as doubles, zeroed once)
-  const size_t n = (size_t)q.nChains * q.d + q.nChains + (size_t)q.nChains * 2 * q.nC * 6 +
-                   (size_t)2 * q.nChains * D.nBlocks + q.nChains;
+  if ((int64_t)D.nBlocks * K > 0x7fffffff) {
+    delete r;
+    return fail(ctx, CPAB_ERR_ARG, "image too large for the launch grid");
+  }
+  // scratch, zeroed once: theta_prop | ll_prop | tables | partial | partial_nfb | stop | done
+  const size_t nck = (size_t)q.nChains * K;
+  const size_t n = nck * q.d + q.nChains + nck * 2 * q.nC * 6 + 2 * nck * D.nBlocks + q.nChains + q.nChains;
   hipError_t e = hipMalloc(&r->scratch, n * sizeof(double));
   if (e == hipSuccess) e = hipMemset(r->scratch, 0, n * sizeof(double));
   if (e != hipSuccess) {
@@ -661,11 +728,13 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
     return fail(ctx, CPAB_ERR_NOMEM, "hipMalloc: %s", hipGetErrorString(e));
   }
   D.theta_prop = r->scratch;
-  D.ll_prop = D.theta_prop + (size_t)q.nChains * q.d;
+  D.ll_prop = D.theta_prop + nck * q.d;
   D.tables = D.ll_prop + q.nChains;
-  D.partial = D.tables + (size_t)q.nChains * 2 * q.nC * 6;
-  D.partial_nfb = D.partial + (size_t)q.nChains * D.nBlocks;
-  D.done = reinterpret_cast<unsigned int*>(D.partial_nfb + (size_t)q.nChains * D.nBlocks);
+  D.partial = D.tables + nck * 2 * q.nC * 6;
+  D.partial_nfb = D.partial + nck * D.nBlocks;
+  D.stop = reinterpret_cast<int64_t*>(D.partial_nfb + nck * D.nBlocks);
+  D.done = reinterpret_cast<unsigned int*>(D.stop + q.nChains);
+  r->host_step.resize(q.nChains);
   *out = r;
   return CPAB_OK;
 }
@@ -673,7 +742,7 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
 int cpab_reg2d_init(cpab_reg2d* r) {
   if (!r) return CPAB_ERR_ARG;
   McmcDesc& D = r->D;
-  const size_t sh = (size_t)mcmc_prop_doubles(D.nC, D.d) * sizeof(double);
+  const size_t sh = (size_t)mcmc_arm_doubles(D.K, D.d) * sizeof(double);
   hipLaunchKernelGGL(k_mcmc_prep, dim3(D.nChains), dim3(256), sh, r->ctx->stream, D, 0);
   HIP_OK(r->ctx, hipGetLastError());
   return reg2d_launch_step(r, r->ctx->stream, 1);
@@ -682,32 +751,29 @@ int cpab_reg2d_init(cpab_reg2d* r) {
 int cpab_reg2d_run(cpab_reg2d* r, int nSteps, int use_graph) {
   if (!r) return CPAB_ERR_ARG;
   cpab_ctx* ctx = r->ctx;
+  McmcDesc& D = r->D;
   if (nSteps < 0) return fail(ctx, CPAB_ERR_ARG, "nSteps must be >= 0");
-  if (!use_graph) {
-    for (int i = 0; i < nSteps; i++) {
-      int rc = reg2d_launch_step(r, ctx->stream, 0);
-      if (rc) return rc;
-    }
-    return CPAB_OK;
-  }
-  if (!r->exec) {
-    // capture kGraphSteps steps on a private stream; replays go to the context's stream
-    HIP_OK(ctx, hipStreamCreate(&r->cap));
-    HIP_OK(ctx, hipStreamBeginCapture(r->cap, hipStreamCaptureModeThreadLocal));
-    int rc = CPAB_OK;
-    for (int i = 0; i < kGraphSteps && rc == CPAB_OK; i++) rc = reg2d_launch_step(r, r->cap, 0);
-    hipGraph_t gph = nullptr;
-    hipError_t e = hipStreamEndCapture(r->cap, &gph);
+  if (nSteps == 0) return CPAB_OK;
+  hipLaunchKernelGGL(k_mcmc_set_stop, dim3((D.nChains + 255) / 256), dim3(256), 0, ctx->stream, D,
+                     (int64_t)nSteps);
+  HIP_OK(ctx, hipGetLastError());
+  if (D.K == 1) return reg2d_launches(r, nSteps, use_graph);  // one step per launch
+  // A launch decides 1..K steps per chain; launch the fewest that could end
+  // the run, read the counters back, repeat.  Launches past a chain's stop
+  // leave it unchanged.
+  std::vector<int64_t>& hs = r->host_step;
+  HIP_OK(ctx, hipMemcpyAsync(hs.data(), D.step, sizeof(int64_t) * D.nChains, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<int64_t> stop(hs.begin(), hs.end());
+  for (auto& v : stop) v += nSteps;
+  int64_t left = nSteps;
+  while (left > 0) {
+    int rc = reg2d_launches(r, (left + D.K - 1) / D.K, use_graph);
     if (rc) return rc;
-    if (e != hipSuccess) return fail(ctx, CPAB_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    r->graph = gph;
-    HIP_OK(ctx, hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
-  }
-  const int full = nSteps / kGraphSteps;
-  for (int i = 0; i < full; i++) HIP_OK(ctx, hipGraphLaunch(r->exec, ctx->stream));
-  for (int i = full * kGraphSteps; i < nSteps; i++) {
-    int rc = reg2d_launch_step(r, ctx->stream, 0);
-    if (rc) return rc;
+    HIP_OK(ctx, hipMemcpyAsync(hs.data(), D.step, sizeof(int64_t) * D.nChains, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    left = 0;
+    for (int c = 0; c < D.nChains; c++) left = stop[c] - hs[c] > left ? stop[c] - hs[c] : left;
   }
   return CPAB_OK;
 }
@@ -715,12 +781,19 @@ int cpab_reg2d_run(cpab_reg2d* r, int nSteps, int use_graph) {
 int cpab_reg2d_last_proposal(cpab_reg2d* r, double* theta_prop, double* ll_prop) {
   if (!r) return CPAB_ERR_ARG;
   cpab_ctx* ctx = r->ctx;
-  if (theta_prop)
-    HIP_OK(ctx, hipMemcpyAsync(theta_prop, r->D.theta_prop, sizeof(double) * r->D.nChains * r->D.d,
-                               hipMemcpyDeviceToDevice, ctx->stream));
+  const McmcDesc& D = r->D;
+  if (theta_prop)  // slot 0 of every chain
+    HIP_OK(ctx, hipMemcpy2DAsync(theta_prop, sizeof(double) * D.d, D.theta_prop, sizeof(double) * D.K * D.d,
+                                 sizeof(double) * D.d, D.nChains, hipMemcpyDeviceToDevice, ctx->stream));
   if (ll_prop)
-    HIP_OK(ctx, hipMemcpyAsync(ll_prop, r->D.ll_prop, sizeof(double) * r->D.nChains, hipMemcpyDeviceToDevice,
+    HIP_OK(ctx, hipMemcpyAsync(ll_prop, D.ll_prop, sizeof(double) * D.nChains, hipMemcpyDeviceToDevice,
                                ctx->stream));
+  return CPAB_OK;
+}
+
+int cpab_reg2d_spec_depth(cpab_reg2d* r, int* K) {
+  if (!r || !K) return CPAB_ERR_ARG;
+  *K = r->D.K;
   return CPAB_OK;
 }
 
@@ -745,15 +818,17 @@ int cpab_prof_read(cpab_ctx* ctx, int64_t* n, double* ms) {
   if (!ctx) return CPAB_ERR_ARG;
   double total = ctx->prof_ms;
   int64_t cnt = ctx->prof_n;
-  for (auto& p : ctx->ev) {
+  for (size_t i = 0; i < ctx->ev.size(); i++) {
+    auto& p = ctx->ev[i];
     HIP_OK(ctx, hipEventSynchronize(p.second));
     float t = 0.f;
     HIP_OK(ctx, hipEventElapsedTime(&t, p.first, p.second));
     total += t;
-    cnt++;
+    cnt += ctx->ev_n[i];
     ctx->ev_free.push_back(p);
   }
   ctx->ev.clear();
+  ctx->ev_n.clear();
   ctx->prof_ms = 0.0;
   ctx->prof_n = 0;
   if (n) *n = cnt;

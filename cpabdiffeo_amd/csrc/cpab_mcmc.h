@@ -1,11 +1,12 @@
 // Device-resident Metropolis sampling of 2D image registration.
 //
-// One step of Metropolis.step (cpab/cpaNd/inference/Metropolis.py:41-85) for
+// Steps of Metropolis.step (cpab/cpaNd/inference/Metropolis.py:41-85) for
 // many independent chains, with the Gaussian SSD likelihood of
-// cpab/cpa2d/model/transformations/register/_ScaleDependentLogLikelihoodGaussian.py:34-109,
-// is ONE kernel launch, k_mcmc_step, with grid (blocks per chain, chains):
+// cpab/cpa2d/model/transformations/register/_ScaleDependentLogLikelihoodGaussian.py:34-109.
+// One launch of k_mcmc_step, grid (K x blocks per chain, chains), evaluates K
+// armed proposals per chain and decides as many steps as stay valid:
 //
-//   prologue  every block loads the chain's armed proposal tables into LDS.
+//   prologue  every block loads its slot's proposal tables into LDS.
 //   body      calc_T_inv of the block's pixels (x = column, y = row); at each
 //             finished point the bilinear sample of the source image and
 //             (warped - target)^2.
@@ -14,12 +15,17 @@
 //             (divided by the pixel count when use_ave_ll, Metropolis.py:58-59),
 //             log_ratio = ll' - ll, accept when log_ratio > 0 or log(u) <
 //             log_ratio (with `inverted` the reference's test log(u) >
-//             log_ratio, Metropolis.py:77-82), advances the chain's step
-//             counter, and arms the next proposal once for the chain:
-//             theta' = theta + L z (z ~ N(0, I), L lower triangular:
-//             Proposal.__call__, cpab/cpaNd/inference/Proposal.py:58-64),
-//             Avees = B theta' (CpaSpace.theta2Avees, _CpaSpace.py:366-376),
-//             sign -1, dt scaling and expm per cell (the calc_T_inv prep).
+//             log_ratio, Metropolis.py:77-82), slot after slot until the
+//             first rejection, advances the step counter by the slots
+//             decided, and arms the next K proposals for the chain.
+//
+// Speculation (K > 1, prefetching in the sense of Brockwell, J. Comput.
+// Graph. Stat. 15(1), 2006): slot m holds the proposal of step s + m on the
+// path where steps s .. s+m-1 are all accepted.  Slots are decided in order;
+// the first rejected slot ends the launch's steps and the later slots are
+// discarded.  Every decided step uses exactly the numbers the K = 1 chain
+// uses (same proposal sums, same random counters), so the chain is the same
+// bit for bit; only idle GPU capacity is spent on the discarded slots.
 // k_mcmc_prep arms the first (noise-free) proposal before the init launch.
 //
 // Random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), keyed by
@@ -70,208 +76,291 @@ struct McmcDesc {
   int nT, nS;
   int use_ave_ll, inverted, trace_len;
   uint32_t seed_lo, seed_hi;
+  int K;                // proposals evaluated per chain and launch (speculation depth)
   const double* B;      // (nC*6, d) row-major
   const double* L;      // (d, d) row-major lower triangular
   const float* src;     // (H, W)
   const float* dst;     // (H, W)
   double* theta;        // (nChains, d)
-  double* theta_prop;   // (nChains, d)
+  double* theta_prop;   // (nChains, K, d): armed proposals, slot m = step + m
   double* ll;           // (nChains,)
   double* ll_prop;      // (nChains,)
   int64_t* n_acc;       // (nChains,)
   int64_t* step;        // (nChains,)
-  double* tables;       // (nChains, 2, nC, 6): the armed proposal's Trels then signed As
-  double* partial;      // (nChains, nBlocks) block sums of squared errors
-  double* partial_nfb;  // (nChains, nBlocks) block counts of rejected proposals
+  int64_t* stop;        // (nChains,) step counter at which the current run ends
+  double* tables;       // (nChains, K, 2, nC, 6): per slot the Trels then the signed As
+  double* partial;      // (nChains, K, nBlocks) block sums of squared errors
+  double* partial_nfb;  // (nChains, K, nBlocks) block counts of rejected Trel proposals
   unsigned int* done;   // (nChains,) blocks finished in the current launch (0 between launches)
   double* trace;        // (nChains, trace_len) or null
-  unsigned long long* nfb;  // (nChains,) rejected Trel proposals, accumulated; or null
-  int nBlocks;          // integration blocks per chain
+  unsigned long long* nfb;  // (nChains,) rejected Trel proposals of decided steps, accumulated; or null
+  int64_t* stats;       // (nChains, 4) or null: proposals evaluated, their rejected Trel
+                        // proposals, launches, steps decided
+  int nBlocks;          // integration blocks per chain and slot
   int ptsPerWave;
   Geom g;
 };
 
-// LDS workspace of the proposal (doubles): z, theta', Avees.
-__host__ __device__ inline int mcmc_prop_doubles(int nC, int d) { return 2 * d + nC * 6; }
+// LDS of the last block's epilogue (doubles): ll' and fallback count per
+// slot, the decided theta, normals and proposals of every slot.
+__host__ __device__ inline int mcmc_finish_doubles(int K, int d) { return 2 * K + d + 2 * K * d; }
+// LDS of k_mcmc_prep (doubles): normals and proposals of every slot.
+__host__ __device__ inline int mcmc_arm_doubles(int K, int d) { return 2 * K * d; }
 
-// The proposal of chain `ch` for the step with counter `step`, from the
-// state theta (LDS or global), by the whole block: theta' = theta + L z,
-// Avees = B theta', and the calc_T_inv tables (Trels then signed As) into
-// D.tables[ch]; theta' into D.theta_prop[ch].  `w` is LDS workspace.
-__device__ inline void mcmc_propose(const McmcDesc& D, int ch, int64_t step, int with_noise, const double* theta,
-                                    double* w) {
-  const int d = D.d, nC = D.nC;
-  double* z = w;
-  double* th = z + d;
-  double* av = th + d;
+// Arm the K proposals that follow `step` from the state theta, by the whole
+// block.  Slot m is the proposal of step + m on the path where every earlier
+// slot is accepted: theta'_0 = theta + L z_step, theta'_m = theta'_{m-1} +
+// L z_{step+m} -- the same sums, in the same order, as the one-step-at-a-time
+// chain forms them (Proposal.__call__, cpab/cpaNd/inference/Proposal.py:58-64).
+// Then Avees = B theta' (CpaSpace.theta2Avees, _CpaSpace.py:366-376), sign -1,
+// dt scaling and expm per cell (the calc_T_inv prep) into D.tables.
+// with_noise == 0 arms slot 0 with theta itself.  z, th: K*d doubles of LDS each.
+__device__ inline void mcmc_arm(const McmcDesc& D, int ch, int64_t step, int with_noise, const double* theta,
+                                double* z, double* th) {
+  const int d = D.d, nC = D.nC, K = D.K;
+  const int nk = with_noise ? K : 1;
+  double* tp = D.theta_prop + (int64_t)ch * K * d;
   if (with_noise) {
-    for (int t = threadIdx.x; 2 * t < d; t += blockDim.x) {
-      const Philox r = philox4x32_10((uint32_t)step, (uint32_t)(step >> 32), (uint32_t)ch, (uint32_t)t, D.seed_lo,
+    const int hd = (d + 1) / 2;
+    for (int idx = threadIdx.x; idx < K * hd; idx += blockDim.x) {
+      const int m = idx / hd, t = idx - m * hd;
+      const int64_t s = step + m;
+      const Philox r = philox4x32_10((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)ch, (uint32_t)t, D.seed_lo,
                                      D.seed_hi);
       const double u1 = 1.0 - u53(r.x0, r.x1), u2 = u53(r.x2, r.x3);
       const double rad = sqrt(-2.0 * log(u1));
       const double ang = 6.283185307179586 * u2;
-      z[2 * t] = rad * cos(ang);
-      if (2 * t + 1 < d) z[2 * t + 1] = rad * sin(ang);
+      z[m * d + 2 * t] = rad * cos(ang);
+      if (2 * t + 1 < d) z[m * d + 2 * t + 1] = rad * sin(ang);
     }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < d; i += blockDim.x) {
-    double v = theta[i];
-    if (with_noise) {
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < K * d; idx += blockDim.x) {
+      const int m = idx / d, i = idx - m * d;
       double acc = 0.0;
-      for (int j = 0; j <= i; j++) acc += D.L[(int64_t)i * d + j] * z[j];
-      v = v + acc;
+      for (int j = 0; j <= i; j++) acc += D.L[(int64_t)i * d + j] * z[m * d + j];
+      th[idx] = acc;
     }
-    th[i] = v;
-    D.theta_prop[(int64_t)ch * d + i] = v;
+    __syncthreads();
+    for (int i = threadIdx.x; i < d; i += blockDim.x) {
+      double v = theta[i];
+      for (int m = 0; m < K; m++) {
+        v = v + th[m * d + i];
+        th[m * d + i] = v;
+        tp[m * d + i] = v;
+      }
+    }
+  } else {
+    for (int i = threadIdx.x; i < d; i += blockDim.x) {
+      th[i] = theta[i];
+      tp[i] = theta[i];
+    }
   }
   __syncthreads();
-  for (int r = threadIdx.x; r < nC * 6; r += blockDim.x) {
-    double acc = 0.0;
-    const double* row = D.B + (int64_t)r * d;
-    for (int j = 0; j < d; j++) acc += row[j] * th[j];
-    av[r] = acc;
-  }
-  __syncthreads();
-  double* T = D.tables + (int64_t)ch * 2 * nC * 6;
-  double* sA = T + nC * 6;
-  for (int c = threadIdx.x; c < nC; c += blockDim.x) {
+  for (int idx = threadIdx.x; idx < nk * nC; idx += blockDim.x) {
+    const int m = idx / nC, c = idx - m * nC;
+    const double* thm = th + m * d;
     double a[6], adt[6], t6[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-      double v = av[c * 6 + k];
+      const double* row = D.B + (int64_t)(c * 6 + k) * d;
+      double acc = 0.0;
+      for (int j = 0; j < d; j++) acc += row[j] * thm[j];
+      double v = acc;
       v *= -1;  // calc_T_inv
       a[k] = v;
       adt[k] = v * D.dt;
     }
     expm_affine2(adt, t6, 12);
+    double* T = D.tables + ((int64_t)ch * K + m) * 2 * nC * 6;
 #pragma unroll
     for (int k = 0; k < 6; k++) {
       T[c * 6 + k] = t6[k];
-      sA[c * 6 + k] = a[k];
+      T[nC * 6 + c * 6 + k] = a[k];
     }
   }
 }
 
-// One block per chain: arm the proposal of the chain's current step.
+// One block per chain: arm the proposals of the chain's current step.
 __global__ void __launch_bounds__(256) k_mcmc_prep(McmcDesc D, int with_noise) {
   extern __shared__ double w[];
   const int ch = blockIdx.x;
-  mcmc_propose(D, ch, D.step[ch], with_noise, D.theta + (int64_t)ch * D.d, w);
+  mcmc_arm(D, ch, D.step[ch], with_noise, D.theta + (int64_t)ch * D.d, w, w + D.K * D.d);
 }
 
-// The last block of the chain: reduce, decide, advance, then arm the next
-// proposal.  `w` is LDS workspace of d + mcmc_prop_doubles doubles.
-__device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int init, double* w) {
-  double* theta_new = w;
-  if (threadIdx.x < 64) {
+// stop = step + n for every chain (the end of a run of n steps).
+__global__ void k_mcmc_set_stop(McmcDesc D, int64_t n) {
+  const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ch < D.nChains) D.stop[ch] = D.step[ch] + n;
+}
+
+// The last block of the chain: reduce every evaluated slot, decide the slots
+// in step order while they stay valid (slot m is valid when slots 0..m-1
+// were accepted), advance, then arm the next K proposals.
+__device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int kact, int init, double* w,
+                                   int* sh) {
+  const int K = D.K, d = D.d;
+  double* llp = w;
+  double* nfs = w + K;
+  double* theta_new = nfs + K;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int m = wv; m < kact; m += nw) {
+    const double* ps = D.partial + ((int64_t)ch * K + m) * D.nBlocks;
+    const double* pn = D.partial_nfb + ((int64_t)ch * K + m) * D.nBlocks;
     double s = 0.0, nf = 0.0;
-    for (int i = threadIdx.x; i < D.nBlocks; i += 64) {
-      s += __hip_atomic_load(D.partial + (int64_t)ch * D.nBlocks + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      nf += __hip_atomic_load(D.partial_nfb + (int64_t)ch * D.nBlocks + i, __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_AGENT);
+    for (int i = lane; i < D.nBlocks; i += 64) {
+      s += __hip_atomic_load(ps + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      nf += __hip_atomic_load(pn + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       s += __shfl_xor(s, o, 64);
       nf += __shfl_xor(nf, o, 64);
     }
-    double llp = s * (-0.5 / (D.sigma * D.sigma));
-    if (D.use_ave_ll) llp = llp / (double)((int64_t)D.H * D.W);
-    bool acc;
-    if (init) {
-      acc = true;
-    } else {
-      const double lr = llp - D.ll[ch];
-      if (lr > 0) {
-        acc = true;
-      } else {
-        const Philox r = philox4x32_10((uint32_t)step, (uint32_t)(step >> 32), (uint32_t)ch, SLOT_ACCEPT,
-                                       D.seed_lo, D.seed_hi);
-        const double lu = log(u53(r.x0, r.x1));
-        acc = D.inverted ? (lu > lr) : (lu < lr);
-      }
-    }
-    for (int i = threadIdx.x; i < D.d; i += 64) {
-      const double v = acc ? D.theta_prop[(int64_t)ch * D.d + i] : D.theta[(int64_t)ch * D.d + i];
-      theta_new[i] = v;
-      D.theta[(int64_t)ch * D.d + i] = v;
-    }
-    if (threadIdx.x == 0) {
-      D.ll_prop[ch] = llp;
-      if (acc) D.ll[ch] = llp;
-      if (!init) {
-        if (acc) D.n_acc[ch] += 1;
-        if (D.trace && step < D.trace_len) D.trace[(int64_t)ch * D.trace_len + step] = D.ll[ch];
-        D.step[ch] = step + 1;
-      }
-      if (D.nfb) D.nfb[ch] += (unsigned long long)nf;
-      D.done[ch] = 0;  // re-arm for the next launch (kernel boundaries order it)
+    if (lane == 0) {
+      double l = s * (-0.5 / (D.sigma * D.sigma));
+      if (D.use_ave_ll) l = l / (double)((int64_t)D.H * D.W);
+      llp[m] = l;
+      nfs[m] = nf;
     }
   }
   __syncthreads();
-  // the next step's proposal, from the state just decided
-  mcmc_propose(D, ch, init ? step : step + 1, 1, theta_new, w + D.d);
+  if (threadIdx.x == 0) {
+    double ll = D.ll[ch];
+    int nd = 0, la = -1;
+    int64_t nacc = 0;
+    double nf_dec = 0.0, nf_all = 0.0;
+    for (int m = 0; m < kact; m++) nf_all += nfs[m];
+    for (int m = 0; m < kact; m++) {
+      bool acc;
+      if (init) {
+        acc = true;
+      } else {
+        const double lr = llp[m] - ll;
+        if (lr > 0) {
+          acc = true;
+        } else {
+          const int64_t s = step + m;
+          const Philox r = philox4x32_10((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)ch, SLOT_ACCEPT, D.seed_lo,
+                                         D.seed_hi);
+          const double lu = log(u53(r.x0, r.x1));
+          acc = D.inverted ? (lu > lr) : (lu < lr);
+        }
+      }
+      nd++;
+      nf_dec += nfs[m];
+      if (acc) {
+        ll = llp[m];
+        la = m;
+        nacc++;
+      }
+      if (!init && D.trace && step + m < D.trace_len) D.trace[(int64_t)ch * D.trace_len + step + m] = ll;
+      if (!acc) break;
+    }
+    D.ll_prop[ch] = llp[nd - 1];
+    D.ll[ch] = ll;
+    if (!init) {
+      D.n_acc[ch] += nacc;
+      D.step[ch] = step + nd;
+    }
+    if (D.nfb) D.nfb[ch] += (unsigned long long)nf_dec;
+    if (D.stats) {
+      int64_t* st = D.stats + (int64_t)ch * 4;
+      st[0] += kact;
+      st[1] += (int64_t)nf_all;
+      st[2] += 1;
+      st[3] += init ? 0 : nd;
+    }
+    D.done[ch] = 0;  // re-arm for the next launch (kernel boundaries order it)
+    sh[0] = la;
+    sh[1] = nd;
+  }
+  __syncthreads();
+  const int la = sh[0], nd = sh[1];
+  for (int i = threadIdx.x; i < d; i += blockDim.x) {
+    const double v = la >= 0 ? D.theta_prop[((int64_t)ch * K + la) * d + i] : D.theta[(int64_t)ch * d + i];
+    theta_new[i] = v;
+    D.theta[(int64_t)ch * d + i] = v;
+  }
+  __syncthreads();
+  double* z = theta_new + d;
+  mcmc_arm(D, ch, init ? step : step + nd, 1, theta_new, z, z + K * d);
 }
 
-// init == 1: evaluate the armed (noise-free) proposal as the current state.
+// One launch evaluates up to K proposals of every chain: grid (K * nBlocks,
+// nChains), block b integrates pixel range b / K of slot b % K, so the
+// blocks of one pixel range (the slowest one holds the domain corner) are
+// dispatched together.  init == 1: evaluate the armed (noise-free) slot 0 as
+// the current state.  A chain whose run has ended (step >= stop) is left as
+// it is.  Every block of a live chain, evaluating or not, counts its arrival,
+// so the last arrival is the only block that touches the chain's state.
 template <int TESS, bool POW2>
 __global__ void __launch_bounds__(256) k_mcmc_step(McmcDesc D, int init) {
   extern __shared__ double lds[];
   __shared__ double red[4], rednf[4];
   __shared__ int last;
+  __shared__ int fin[2];
   const int ch = blockIdx.y;
+  const int K = D.K;
+  const int slot = (int)(blockIdx.x % (unsigned)K);
+  const int bx = (int)(blockIdx.x / (unsigned)K);
   const int nC = D.nC;
   const int64_t step = D.step[ch];
-  const double* tab = D.tables + (int64_t)ch * 2 * nC * 6;
-  for (int i = threadIdx.x; i < 2 * nC * 6; i += blockDim.x) lds[i] = tab[i];
-  __syncthreads();
-  const double* Ttab = lds;
-  const double* Atab = lds + nC * 6;
-  const Geom& g = D.g;
-  const int64_t nPts = (int64_t)D.H * D.W;
-  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int64_t begin = wave * (int64_t)D.ptsPerWave;
-  const int64_t end = begin + D.ptsPerWave < nPts ? begin + D.ptsPerWave : nPts;
-  const int nT = D.nT, nS = D.nS;
-  const double h = D.h;
+  int kact = 1;
+  if (!init) {
+    const int64_t left = D.stop[ch] - step;
+    if (left <= 0) return;  // grid-uniform per chain: no block of this chain counts
+    kact = left < K ? (int)left : K;
+  }
   double acc = 0.0;
   int nfb = 0;
-  if (begin < nPts) {
-    int64_t next = begin;
-    double p[2];
-    int c = 0, t = 0, k = 0;
-    int64_t my = 0;
-    bool active = false;
-    for (;;) {
-      if (active && t >= nT) {
-        const float w = sample2(D.src, D.H, D.W, p[0], p[1]);
-        const double e = (double)w - (double)D.dst[my];
-        acc += e * e;
-        active = false;
-      }
-      const uint64_t idle = ballot(!active);
-      if (idle != 0 && next < end) {
-        if (!active) {
-          const int64_t cand = next + rank_in(idle);
-          if (cand < end) {
-            my = cand;
-            active = true;
-            p[0] = (double)(my % D.W);
-            p[1] = (double)(my / D.W);
-            c = cell2<TESS, POW2>(p, g);
-            t = 0;
-            k = 0;
-          }
+  if (slot < kact) {
+    const double* tab = D.tables + ((int64_t)ch * K + slot) * 2 * nC * 6;
+    for (int i = threadIdx.x; i < 2 * nC * 6; i += blockDim.x) lds[i] = tab[i];
+    __syncthreads();
+    const double* Ttab = lds;
+    const double* Atab = lds + nC * 6;
+    const Geom& g = D.g;
+    const int64_t nPts = (int64_t)D.H * D.W;
+    const int64_t wave = (int64_t)bx * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t begin = wave * (int64_t)D.ptsPerWave;
+    const int64_t end = begin + D.ptsPerWave < nPts ? begin + D.ptsPerWave : nPts;
+    const int nT = D.nT, nS = D.nS;
+    const double h = D.h;
+    if (begin < nPts) {
+      int64_t next = begin;
+      double p[2];
+      int c = 0, t = 0, k = 0;
+      int64_t my = 0;
+      bool active = false;
+      for (;;) {
+        if (active && t >= nT) {
+          const float w = sample2(D.src, D.H, D.W, p[0], p[1]);
+          const double e = (double)w - (double)D.dst[my];
+          acc += e * e;
+          active = false;
         }
-        next += __popcll(idle);
+        const uint64_t idle = ballot(!active);
+        if (idle != 0 && next < end) {
+          if (!active) {
+            const int64_t cand = next + rank_in(idle);
+            if (cand < end) {
+              my = cand;
+              active = true;
+              p[0] = (double)(my % D.W);
+              p[1] = (double)(my / D.W);
+              c = cell2<TESS, POW2>(p, g);
+              t = 0;
+              k = 0;
+            }
+          }
+          next += __popcll(idle);
+        }
+        if (ballot(active) == 0) break;
+        if (!active || t >= nT) continue;
+        do {
+          if (trip<2, TESS, POW2>(p, c, k, nfb, Ttab, Atab, h, nS, g)) t++;
+        } while (ballot(t >= nT) == 0);
       }
-      if (ballot(active) == 0) break;
-      if (!active || t >= nT) continue;
-      do {
-        if (trip<2, TESS, POW2>(p, c, k, nfb, Ttab, Atab, h, nS, g)) t++;
-      } while (ballot(t >= nT) == 0);
     }
   }
   // block sums (fixed order), published with release; the last block
@@ -289,20 +378,21 @@ __global__ void __launch_bounds__(256) k_mcmc_step(McmcDesc D, int init) {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double sum = 0.0, sumnf = 0.0;
-    for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
-      sum += red[i];
-      sumnf += rednf[i];
+    if (slot < kact) {
+      double sum = 0.0, sumnf = 0.0;
+      for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+        sum += red[i];
+        sumnf += rednf[i];
+      }
+      const int64_t at = ((int64_t)ch * K + slot) * D.nBlocks + bx;
+      __hip_atomic_store(D.partial + at, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(D.partial_nfb + at, sumnf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __hip_atomic_store(D.partial + (int64_t)ch * D.nBlocks + blockIdx.x, sum, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(D.partial_nfb + (int64_t)ch * D.nBlocks + blockIdx.x, sumnf, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int prev = __hip_atomic_fetch_add(D.done + ch, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (prev == (unsigned int)(D.nBlocks - 1));
+    last = (prev == (unsigned int)(K * D.nBlocks - 1));
   }
   __syncthreads();
-  if (last) mcmc_finish(D, ch, step, init, lds);  // block-uniform branch
+  if (last) mcmc_finish(D, ch, step, kact, init, lds, fin);  // block-uniform branch
 }
 
 }  // namespace cpab

@@ -37,7 +37,8 @@ class RegDesc(ctypes.Structure):
                 ("seed", ctypes.c_uint64), ("B", ctypes.c_void_p), ("Lprop", ctypes.c_void_p),
                 ("src", ctypes.c_void_p), ("dst", ctypes.c_void_p), ("theta", ctypes.c_void_p),
                 ("ll", ctypes.c_void_p), ("n_accepted", ctypes.c_void_p), ("step", ctypes.c_void_p),
-                ("trace", ctypes.c_void_p), ("trace_len", ctypes.c_int), ("n_fallback", ctypes.c_void_p)]
+                ("trace", ctypes.c_void_p), ("trace_len", ctypes.c_int), ("n_fallback", ctypes.c_void_p),
+                ("spec_depth", ctypes.c_int), ("stats", ctypes.c_void_p)]
 
 
 def pixel_basis(space_unit, n):
@@ -84,10 +85,12 @@ class DeviceMetropolis:
     """nChains Metropolis chains of one RegistrationProblem, all on the GPU.
 
     The API follows Metropolis (set_theta / step / run / record); theta has
-    shape (nChains, d)."""
+    shape (nChains, d).  spec_depth: proposals evaluated per chain and launch
+    (1: one step per launch; K > 1: speculative prefetch of the next K steps,
+    the same chain bit for bit; 0: chosen by the library)."""
 
     def __init__(self, problem, nChains=1, scale=0.01, diagonal=False, use_ave_ll=True,
-                 inverted_acceptance=False, seed=0, trace_len=0, count_fallbacks=False):
+                 inverted_acceptance=False, seed=0, trace_len=0, count_fallbacks=False, spec_depth=1):
         torch = require_gpu()
         self.problem = problem
         self.nChains = nChains
@@ -114,6 +117,8 @@ class DeviceMetropolis:
         self.trace_len = trace_len
         self.trace = torch.zeros((nChains, max(trace_len, 1)), dtype=f64, device=dev)
         self.n_fallback = torch.zeros(nChains, dtype=torch.int64, device=dev)
+        # proposals evaluated, their rejected Trel proposals, launches, steps decided
+        self.stats = torch.zeros((nChains, 4), dtype=torch.int64, device=dev)
         p = problem.params
         T = sp.tessellation
         desc = RegDesc(nChains, d, T.type, sp.nC, int(T.nCs[0]), int(T.nCs[1]), float(T.incs[0]),
@@ -122,12 +127,16 @@ class DeviceMetropolis:
                        int(seed), self._B.data_ptr(), self._L.data_ptr(), self._src.data_ptr(),
                        self._dst.data_ptr(), self.theta.data_ptr(), self.ll.data_ptr(), self.n_accepted.data_ptr(),
                        self.step_counter.data_ptr(), self.trace.data_ptr() if trace_len else None, int(trace_len),
-                       self.n_fallback.data_ptr() if count_fallbacks else None)
+                       self.n_fallback.data_ptr() if count_fallbacks else None, int(spec_depth),
+                       self.stats.data_ptr())
         self._ctx = Context.get(0)
         self._ctx.bind_stream(torch.cuda.current_stream().cuda_stream)
         h = ctypes.c_void_p()
         check(lib().cpab_reg2d_create(self._ctx.handle, ctypes.byref(desc), ctypes.byref(h)), self._ctx.handle)
         self._h = h
+        k = ctypes.c_int()
+        check(lib().cpab_reg2d_spec_depth(h, ctypes.byref(k)), self._ctx.handle)
+        self.spec_depth = k.value
         self.nSteps = 0
         self.record = {"runs": []}
 
@@ -148,7 +157,8 @@ class DeviceMetropolis:
         self.run(1)
 
     def run(self, N, use_graph=True):
-        """N steps of every chain (Metropolis.run, :104-151), no host sync."""
+        """N steps of every chain (Metropolis.run, :104-151).  With spec_depth 1
+        this only enqueues; with speculation it returns once the steps are done."""
         self._bind()
         t0 = time.perf_counter()
         check(lib().cpab_reg2d_run(self._h, int(N), 1 if use_graph else 0), self._ctx.handle)
@@ -176,3 +186,72 @@ class DeviceMetropolis:
             self.close()
         except Exception:
             pass
+
+
+class HostMetropolis:
+    """The plain path, one GPU call sequence per proposal, structured as the
+    reference's loop: Metropolis.step (cpab/cpaNd/inference/Metropolis.py:41-102)
+    on the host, the proposal theta' = theta + L z and Avees = B theta' in
+    numpy (Proposal.py:29-64, _CpaSpace.py:366-376), update_pat, then
+    calc_T_inv, remap and the SSD reduction on the GPU and a blocking copy of
+    the log-likelihood to the host (the reference's ``.get()``,
+    register/gpu/gaussian.py:15-46).  Random numbers come from numpy, so its
+    chain is not the device chain; it is the baseline the batched device
+    paths are measured against."""
+
+    def __init__(self, problem, scale=0.01, use_ave_ll=True, inverted_acceptance=False, seed=0):
+        torch = require_gpu()
+        self.problem = problem
+        sp = problem.space
+        self.d = problem.space_unit.d
+        C = scale ** 2 * problem.cov
+        self.Lprop = np.linalg.cholesky(C + 1e-12 * np.trace(C) / self.d * np.eye(self.d))
+        self.use_ave_ll = use_ave_ll
+        self.inverted = inverted_acceptance
+        self.rng = np.random.default_rng(seed)
+        n = problem.n
+        self.pts = torch.from_numpy(sp.tessellation.dense_grid_img()).cuda()
+        self.inv = torch.zeros_like(self.pts)
+        self.src = torch.from_numpy(problem.src).cuda()
+        self.dst = torch.from_numpy(problem.dst).cuda()
+        self.warped = torch.zeros((n, n), dtype=torch.float32, device="cuda")
+        self.ll_dev = torch.zeros(1, dtype=torch.float64, device="cuda")
+        self.ctx = Context.get(0)
+        self.nSteps = 0
+        self.n_accepted = 0
+
+    def log_likelihood(self, theta):
+        torch = require_gpu()
+        p = self.problem
+        p.space.update_pat(p.B @ theta)
+        p.space.calc_T_inv(pts=self.pts, out=self.inv, **p.params)
+        self.ctx.bind_stream(torch.cuda.current_stream().cuda_stream)
+        n = p.n
+        self.ctx.call("cpab_remap2d", self.inv.data_ptr(), self.src.data_ptr(), self.warped.data_ptr(), n, n)
+        self.ctx.call("cpab_ssd_ll", self.warped.data_ptr(), self.dst.data_ptr(), n * n, float(p.sigma),
+                      self.ll_dev.data_ptr())
+        ll = float(self.ll_dev.cpu().numpy()[0])
+        return ll / (n * n) if self.use_ave_ll else ll
+
+    def set_theta(self, theta):
+        self.theta = np.array(theta, dtype=np.float64)
+        self.ll = self.log_likelihood(self.theta)
+
+    def step(self):
+        prop = self.theta + self.Lprop @ self.rng.standard_normal(self.d)
+        llp = self.log_likelihood(prop)
+        lr = llp - self.ll
+        if lr > 0:
+            acc = True
+        else:
+            lu = np.log(self.rng.random())
+            acc = (lu > lr) if self.inverted else (lu < lr)
+        if acc:
+            self.theta, self.ll = prop, llp
+            self.n_accepted += 1
+        self.nSteps += 1
+        return acc
+
+    def run(self, N):
+        for _ in range(N):
+            self.step()

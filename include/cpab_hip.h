@@ -171,7 +171,14 @@ typedef struct {
   double *trace;        /* optional (nChains, trace_len): ll after each step */
   int trace_len;
   uint64_t *n_fallback; /* optional (nChains,): accumulates the integration steps
-                           whose Trel proposal left its cell (RK2 redone) */
+                           whose Trel proposal left its cell (RK2 redone), over the
+                           decided steps */
+  int spec_depth;       /* proposals evaluated per chain and launch: 1 = one step per
+                           launch; K > 1 = speculative prefetch of the next K steps on
+                           the all-accepted path (same chain bit for bit); 0 = chosen
+                           by the library (about 2^20 points per launch, at most 16) */
+  int64_t *stats;       /* optional (nChains, 4), accumulated: proposals evaluated,
+                           their rejected Trel proposals, launches, steps decided */
 } cpab_reg2d_desc;
 
 typedef struct cpab_reg2d cpab_reg2d;
@@ -179,14 +186,19 @@ typedef struct cpab_reg2d cpab_reg2d;
 int cpab_reg2d_create(cpab_ctx *ctx, const cpab_reg2d_desc *desc, cpab_reg2d **out);
 /* ll of the current theta of every chain (Metropolis.set_theta, :25-40). */
 int cpab_reg2d_init(cpab_reg2d *r);
-/* nSteps Metropolis steps of every chain (one kernel launch per step);
- * use_graph != 0 replays a captured 16-step hipGraph instead of launching
- * (the remainder of nSteps/16 is launched directly). */
+/* nSteps Metropolis steps of every chain.  A launch decides one step per
+ * chain (spec_depth 1) or 1..K (speculation); use_graph != 0 replays a
+ * captured 16-launch hipGraph instead of launching (the remainder is
+ * launched directly).  With speculation the call reads the step counters
+ * back between batches of launches, so it returns after the steps are done;
+ * with spec_depth 1 it only enqueues. */
 int cpab_reg2d_run(cpab_reg2d *r, int nSteps, int use_graph);
 /* theta_prop (nChains x d): the proposal the next step will evaluate (armed
  * at the end of every step); ll_prop (nChains): the log-likelihood of the
  * proposal evaluated by the last step.  Either pointer may be NULL. */
 int cpab_reg2d_last_proposal(cpab_reg2d *r, double *theta_prop, double *ll_prop);
+/* The speculation depth in use (spec_depth as resolved at create). */
+int cpab_reg2d_spec_depth(cpab_reg2d *r, int *K);
 int cpab_reg2d_destroy(cpab_reg2d *r);
 
 /* Kernel timing with HIP events on the context's stream (no reference

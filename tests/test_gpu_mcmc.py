@@ -33,12 +33,13 @@ def test_target_image_matches_oracle(problem, orc):
     assert abs(noise.std() - 0.02) < 0.002  # the rest is the N(0, 0.02^2) noise
 
 
-def test_chain_matches_cpu_reference(problem, orc, gpu):
+@pytest.mark.parametrize("spec", [1, 4])
+def test_chain_matches_cpu_reference(problem, orc, gpu, spec):
     from cpabdiffeo_amd.registration import DeviceMetropolis
-    say("chain vs CPU reference, 2 chains x 12 steps")
+    say("chain vs CPU reference, 2 chains x 12 steps, spec_depth", spec)
     p = problem
     nsteps = 12
-    mh = DeviceMetropolis(p, nChains=2, scale=0.05, seed=11, trace_len=nsteps)
+    mh = DeviceMetropolis(p, nChains=2, scale=0.05, seed=11, trace_len=nsteps, spec_depth=spec)
     theta0 = np.zeros(mh.d)
     mh.set_theta(theta0)
     ll0 = log_likelihood(orc, p.space, p.B, theta0, p.src, p.dst, p.sigma, p.params)
@@ -91,3 +92,35 @@ def test_fallback_counter_matches_calc_T(problem, orc, gpu):
     _, nfb, _ = orc.calc_T(p.space.tessellation.geom(), p.space.tessellation.dense_grid_img(), T, sA,
                            p.params["dt"], p.params["nTimeSteps"], p.params["nStepsODEsolver"], probe=True)
     assert abs(got - int(nfb.sum())) <= 0.01 * nfb.sum() + 2
+
+
+@pytest.mark.parametrize("nChains,spec,graph", [(1, 8, False), (1, 8, True), (3, 5, True), (1, 0, True)])
+def test_speculation_is_the_same_chain(problem, gpu, nChains, spec, graph):
+    """K proposals per launch on the all-accepted path, decided in order:
+    the chain (theta, trace, accept count, fallback count) is bit-identical
+    to one step per launch, across rejections and run boundaries."""
+    import torch
+    from cpabdiffeo_amd.registration import DeviceMetropolis
+    say("speculation", nChains, spec, graph)
+    p = problem
+    out = []
+    for k in (1, spec):
+        mh = DeviceMetropolis(p, nChains=nChains, scale=0.05, seed=9, trace_len=60, count_fallbacks=True,
+                              spec_depth=k)
+        if k == spec and spec != 1:
+            assert mh.spec_depth > 1
+        mh.set_theta(np.zeros(mh.d))
+        for n in (23, 1, 36):  # run boundaries inside launches
+            mh.run(n, use_graph=graph)
+        torch.cuda.synchronize()
+        out.append([x.cpu().numpy().copy() for x in (mh.theta, mh.trace, mh.n_accepted, mh.n_fallback,
+                                                       mh.step_counter, mh.stats)])
+        mh.close()
+    a, b = out
+    for i in range(5):
+        assert np.array_equal(a[i], b[i]), i
+    assert (a[4] == 60).all()
+    st = b[5]
+    assert (st[:, 3] == 60).all()  # steps decided
+    assert (st[:, 0] >= 60).all()  # proposals evaluated
+    assert 0 < a[2].min() and a[2].max() < 60  # accepts and rejects both happened
