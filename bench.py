@@ -41,6 +41,17 @@ F_FAST = 2 * F_LOOKUP + F_AFFINE           # 38: one accepted Trel step (:389-39
 F_SUB = F_LOOKUP + 2 * F_AFFINE + 6 + 4    # 41: one RK2 substep (:179-194)
 F_REMAP_SSD = 16         # bilinear sample (13) + squared error accumulation (3), per pixel
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
+# HBM bytes per launch of the step kernel at 16 proposals per launch, from the
+# FETCH_SIZE / WRITE_SIZE counter runs scaled by a plain copy (committed summary).
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_mcmc_step_registration256_spec16.md")
+
+
+def measured_traffic(spec_depth):
+    import re
+    if spec_depth != 16 or not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    m = re.search(r"traffic_bytes_per_launch = (\d+)", open(TRAFFIC_SUMMARY).read())
+    return int(m.group(1)) if m else None
 
 
 def log(*a):
@@ -113,7 +124,22 @@ def main():
     ap.add_argument("--cpu-proposals", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-seq", action="store_true", help="skip the one-step-per-launch comparison chain")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy 5 times (the reference point for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.empty(64 * 2 ** 20, dtype=torch.float32, device="cuda").fill_(1.0)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read_per_dispatch": x.numel() * 4,
+                          "copy_bytes_written_per_dispatch": x.numel() * 4}), flush=True)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
@@ -159,10 +185,12 @@ def main():
 
     log("bench: warmup %d steps, then the timed %d-step chain (speculative prefetch)" % (args.warmup, args.steps))
     spec = chain(0, args.warmup, args.steps, True)
-    log("bench: the same chain one step per launch")
-    seq = chain(1, min(args.warmup, 200), args.steps, False)
-    same_chain = bool(np.array_equal(spec["theta"], seq["theta"]) and spec["ll"] == seq["ll"]
-                      and spec["nacc"] == seq["nacc"] and spec["nfb"] == seq["nfb"])
+    seq = None
+    if not args.no_seq:
+        log("bench: the same chain one step per launch")
+        seq = chain(1, min(args.warmup, 200), args.steps, False)
+        same_chain = bool(np.array_equal(spec["theta"], seq["theta"]) and spec["ll"] == seq["ll"]
+                          and spec["nacc"] == seq["nacc"] and spec["nfb"] == seq["nfb"])
     elapsed = spec["elapsed"]
     proposals_per_sec = args.steps / elapsed
     nacc = spec["nacc"]
@@ -220,11 +248,11 @@ def main():
                    "points_per_proposal": npts, "graph_replay": use_graph, "spec_depth": spec["K"],
                    "accept_ratio": nacc / float(args.steps)},
         "points_per_sec": proposals_per_sec * npts,
-        "one_step_per_launch": {"proposals_per_sec": args.steps / seq["elapsed"],
-                                "speedup_of_speculation": seq["elapsed"] / elapsed,
-                                "same_chain_bit_for_bit": same_chain},
+        "one_step_per_launch": None if seq is None else {
+            "proposals_per_sec": args.steps / seq["elapsed"], "speedup_of_speculation": seq["elapsed"] / elapsed,
+            "same_chain_bit_for_bit": same_chain},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(spec["K"]),
                      "kernel": "k_mcmc_step (K proposals: calc_T_inv + remap + SSD; accept; arm next)",
                      "kernel_us": kern_s * 1e6, "launches": launches,
                      "proposals_evaluated_per_launch": st[0] / launches,

@@ -106,6 +106,8 @@ int check_nC(cpab_ctx* ctx, int dim, int tess, int nC, int nC0, int nC1, int nC2
   if (dim >= 2) boxes *= nC1;
   if (dim >= 3) boxes *= nC2;
   const int64_t want = boxes * (tess == TESS_II ? 1 : (dim == 2 ? 4 : 5));
+  // the lookups form cell ids with 24-bit multiplies
+  if (want >= (1 << 24)) return fail(ctx, CPAB_ERR_UNSUPPORTED, "tessellations of 2^24 cells or more are not supported");
   if (nC != want)
     return fail(ctx, CPAB_ERR_ARG, "nC=%d does not match the tessellation (%lld cells)", nC, (long long)want);
   return CPAB_OK;

@@ -38,6 +38,13 @@ struct Geom {
 };
 
 __device__ __forceinline__ double dmin(double a, double b) { return __builtin_fmin(a, b); }
+
+// Product of two non-negative ints below 2^24 (cell and box indices are far
+// below that): the masks let the compiler use v_mul_u32_u24 (full rate)
+// instead of v_mul_lo_u32.
+__device__ __forceinline__ int mul24(int a, int b) {
+  return (int)(((unsigned)a & 0xffffffu) * ((unsigned)b & 0xffffffu));
+}
 __device__ __forceinline__ double dmax(double a, double b) { return __builtin_fmax(a, b); }
 
 // fmod(x, y) for y > 0, exact (the IEEE remainder with the sign of x).
@@ -109,14 +116,14 @@ template <int TESS, bool POW2>
 __device__ __forceinline__ int cell2(const double* p, const Geom& g) {
   if (TESS == TESS_II) {
     return axis_box<POW2>(p[0], g.nC0, g.inc_x, g.rinc_x) +
-           axis_box<POW2>(p[1], g.nC1, g.inc_y, g.rinc_y) * g.nC0;
+           mul24(axis_box<POW2>(p[1], g.nC1, g.inc_y, g.rinc_y), g.nC0);
   }
   const double p0 = dmin(g.top_x, dmax(0.0, p[0]));
   const double p1 = dmin(g.top_y, dmax(0.0, p[1]));
   double x, y;
   const int i = axis_simplex<POW2>(p0, g.nC0, g.inc_x, g.rinc_x, x);
   const int j = axis_simplex<POW2>(p1, g.nC1, g.inc_y, g.rinc_y, y);
-  int c = (i + j * g.nC0) * 4;  // triangles: 0 top, 1 right, 2 bottom, 3 left
+  int c = (i + mul24(j, g.nC0)) * 4;  // triangles: 0 top, 1 right, 2 bottom, 3 left
   const double px = p[0], py = p[1];
   const bool left = px <= 0.0, right = px >= g.wall_x, up = py <= 0.0, down = py >= g.wall_y;
   if (left | right | up | down) {  // rare: outside the domain
@@ -147,7 +154,7 @@ __device__ __forceinline__ int cell3(const double* p, const Geom& g) {
     const int a = axis_box<POW2>(p[0], g.nC0, g.inc_x, g.rinc_x);
     const int b = axis_box<POW2>(p[1], g.nC1, g.inc_y, g.rinc_y);
     const int cc = axis_box<POW2>(p[2], g.nC2, g.inc_z, g.rinc_z);
-    return a + b * g.nC0 + cc * g.nC1 * g.nC0;
+    return a + mul24(b, g.nC0) + mul24(cc, mul24(g.nC1, g.nC0));
   }
   const double p0 = dmin(g.top_x, dmax(0.0, p[0]));
   const double p1 = dmin(g.top_y, dmax(0.0, p[1]));
@@ -156,7 +163,7 @@ __device__ __forceinline__ int cell3(const double* p, const Geom& g) {
   const int i = axis_simplex<POW2>(p0, g.nC0, g.inc_x, g.rinc_x, x);
   const int j = axis_simplex<POW2>(p1, g.nC1, g.inc_y, g.rinc_y, y);
   const int k = axis_simplex<POW2>(p2, g.nC2, g.inc_z, g.rinc_z, z);
-  int c = (i + j * g.nC0 + k * g.nC1 * g.nC0) * 5;
+  int c = mul24(i + mul24(j, g.nC0) + mul24(k, mul24(g.nC1, g.nC0)), 5);
   // alternating orientation: flip when the parity of i + j + k is odd
   const bool tf = ((i + j + k) & 1) != 0;
   if (tf) {

@@ -223,7 +223,7 @@ __device__ __forceinline__ bool trip(double* p, int& c, int& k, int& nfb, const 
                                      const double* Atab, double h, int nS, const Geom& g) {
   constexpr int NE = DIM * (DIM + 1);
   const bool fast = (k == 0);
-  const double* M = (fast ? Ttab : Atab) + c * NE;
+  const double* M = (fast ? Ttab : Atab) + __umul24((unsigned)c, (unsigned)NE);
   double y[DIM];
   affine<DIM>(y, M, p);
   if (!fast) {  // RK2 midpoint substep (solveODE, calc_2Dto2D.cu:182-194)
