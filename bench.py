@@ -40,6 +40,9 @@ F_AFFINE = 8             # A_times_b_affine (:27-28)
 F_FAST = 2 * F_LOOKUP + F_AFFINE           # 38: one accepted Trel step (:389-391)
 F_SUB = F_LOOKUP + 2 * F_AFFINE + 6 + 4    # 41: one RK2 substep (:179-194)
 F_REMAP_SSD = 16         # bilinear sample (13) + squared error accumulation (3), per pixel
+# one substep of the adjoint sweep as written in cpab_adjoint.h: lookup 15, affine 8,
+# midpoint 6, g2 2, L^T g2 6, g1 4, lambda update 10, dJ/dA accumulation 20
+F_ADJ_SUB = 71
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
 # HBM bytes per launch of the step kernel at 16 proposals per launch, from the
 # FETCH_SIZE / WRITE_SIZE counter runs scaled by a plain copy (committed summary).
@@ -113,6 +116,47 @@ def config1_extra(torch, ctx):
             "points_per_sec": len(pts) / wall, "ms_per_call": wall * 1e3, "kernel_us": kms * 1e3,
             "fallback_steps": fb, "alg_tflops": flops / (kms * 1e-3) / 1e12,
             "frac_fp64_peak": flops / (kms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+
+
+def config4_extra(torch, ctx, steps=200, lr=3e-4):
+    """Gradient descent on 64 pairs of 128x128 blob images (one theta per pair),
+    gradient by the fused SSD adjoint; plus a central-difference check."""
+    import numpy as np
+    from cpabdiffeo_amd import adjoint
+    prob = adjoint.PairsProblem(n=128, nInst=64, seed=0)
+    adjoint.gradient_descent(prob, 2, lr)  # workspace, warm clocks
+    torch.cuda.synchronize()
+    ctx.prof_enable(True)
+    ctx.prof_read()
+    t0 = time.perf_counter()
+    theta, hist = adjoint.gradient_descent(prob, steps, lr)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    cnt, ms = ctx.prof_read()
+    ctx.prof_enable(False)
+    h = hist.cpu().numpy()
+    npts = prob.nInst * prob.n * prob.n
+    p = prob.params
+    sub = p["nTimeSteps"] * p["nStepsODEsolver"]
+    flops = npts * sub * (2 * F_SUB + F_ADJ_SUB)
+    kus = ms / cnt * 1e3
+    # central differences (h = 1e-6, FP64 loss) along one random direction, every pair
+    rng = np.random.default_rng(0)
+    v = torch.from_numpy(rng.standard_normal(theta.shape)).cuda()
+    th = theta.clone().requires_grad_(True)
+    loss = prob.loss(th)
+    loss.sum().backward()
+    an = (th.grad * v).sum(1)
+    with torch.no_grad():
+        fd = (prob.loss(theta + 1e-6 * v) - prob.loss(theta - 1e-6 * v)) / 2e-6
+    rel = ((fd - an).abs() / an.abs().clamp_min(1e-12)).max().item()
+    return {"workload": "64 pairs of 128x128 fp32 blob images, nC=64 tess I, one theta per pair, %d gradient-descent "
+                        "steps (lr %g), calc_T_simple t=1 100x10 fp64, SSD adjoint" % (steps, lr),
+            "ms_per_step": el / steps * 1e3, "steps_per_sec": steps / el, "point_gradients_per_sec": npts * steps / el,
+            "adjoint_kernel_us": kus, "adjoint_alg_tflops": flops / (kus * 1e-6) / 1e12,
+            "adjoint_frac_fp64_peak": flops / (kus * 1e-6) / 1e12 / FP64_PEAK_TFLOPS,
+            "loss_mean_first": float(h[0].mean()), "loss_mean_last": float(h[-1].mean()),
+            "fd_max_rel_err": rel}
 
 
 def main():
@@ -206,10 +250,12 @@ def main():
     flops_per_launch = (st[0] * npts * (100 * F_FAST + F_REMAP_SSD) + st[1] * 10 * F_SUB) / launches
     achieved = flops_per_launch / kern_s / 1e12
 
-    extra = None
+    extra = gd = None
     if not args.no_extra:
         log("bench: config-1 calc_T_fwd extra")
         extra = config1_extra(torch, ctx)
+        log("bench: config-4 gradient descent extra")
+        gd = config4_extra(torch, ctx)
 
     cpu = None
     if not args.no_cpu:
@@ -261,6 +307,7 @@ def main():
                      "fallback_steps_per_launch": st[1] / launches},
         "cpu_baseline": cpu,
         "calc_T_fwd_512x512": extra,
+        "gradient_descent_64x128x128": gd,
     }
     print(json.dumps(line), flush=True)
 

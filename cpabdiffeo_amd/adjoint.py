@@ -1,0 +1,205 @@
+"""Gradients of the CPA transform and of the image SSD loss with respect to
+theta, from the hand-written adjoint kernels (cpab_calc_T_simple_vjp,
+cpab_ssd_grad2d), wrapped as torch.autograd.Functions.
+
+The transform differentiated is calc_T_simple's RK2 map
+(cpab/gpu/calc_2Dto2D.cu:412-445), the map whose forward sensitivity the
+reference computes (solveODE2, calc_2Dto2D.cu:202-319; CpaSpace.calc_grad_theta
+here).  theta enters through Avees = B theta (CpaSpace.theta2Avees,
+cpab/cpaNd/_CpaSpace.py:366-376) and the signed tables sign * Avees, so
+dL/dtheta = sign * B^T dL/dA.
+
+torch is plumbing here: tensors carry device memory and the autograd tape;
+the derivative is computed by the library.
+"""
+from .calcs import context_for
+from .gpuarray import require_gpu
+from .tessellation import tess_code
+
+
+def _geom2d(space):
+    T = space.tessellation
+    if space.dim_domain != 2:
+        raise NotImplementedError("the adjoint kernels are 2D")
+    return tess_code(space.tess), space.nC, int(T.nCs[0]), int(T.nCs[1]), float(T.incs[0]), float(T.incs[1])
+
+
+def _params(params):
+    return float(params["dt"]), int(params["nTimeSteps"]), int(params["nStepsODEsolver"])
+
+
+def _signed_tables(torch, theta, B, sign, nC):
+    Av = theta @ B.t().contiguous()
+    return (float(sign) * Av).reshape(theta.shape[0], nC, 6).contiguous()
+
+
+def calc_T_simple_vjp(space, pts, lam, As, params, pos_out=None):
+    """gradA (nInst, nC, 6) = sum_x lam(x)^T dT(x)/dA for the signed tables As
+    (nInst, nC, 6); pts, lam: (nInst, N, 2) device float64."""
+    torch = require_gpu()
+    tess, nC, nC0, nC1, ix, iy = _geom2d(space)
+    dt, nT, nS = _params(params)
+    if As.dim() == 2:
+        As = As.unsqueeze(0)
+    if pts.dim() == 2:
+        pts, lam = pts.unsqueeze(0), lam.unsqueeze(0)
+    nInst, N = pts.shape[0], pts.shape[1]
+    if tuple(As.shape) != (nInst, nC, 6) or tuple(lam.shape) != tuple(pts.shape) or pts.shape[2] != 2:
+        raise ValueError((tuple(As.shape), tuple(pts.shape), tuple(lam.shape)))
+    gradA = torch.empty((nInst, nC, 6), dtype=torch.float64, device=pts.device)
+    ctx = context_for(pts)
+    ctx.call("cpab_calc_T_simple_vjp", tess, nC, nInst, pts.contiguous().data_ptr(), lam.contiguous().data_ptr(),
+             As.contiguous().data_ptr(), dt, nT, nS, int(N), nC0, nC1, ix, iy,
+             pos_out.data_ptr() if pos_out is not None else None, gradA.data_ptr())
+    return gradA
+
+
+def ssd_loss_and_grad(space, As, src, dst, params, pos_out=None):
+    """loss (nInst,) = 0.5 sum_x (src_i(T_i(x)) - dst_i(x))^2 over the pixel
+    grid and gradA (nInst, nC, 6) = dloss/dA, for signed tables As
+    (nInst, nC, 6) and float32 images src, dst (nInst, H, W)."""
+    torch = require_gpu()
+    tess, nC, nC0, nC1, ix, iy = _geom2d(space)
+    dt, nT, nS = _params(params)
+    nInst, H, W = src.shape
+    if tuple(dst.shape) != (nInst, H, W) or tuple(As.shape) != (nInst, nC, 6):
+        raise ValueError((tuple(src.shape), tuple(dst.shape), tuple(As.shape)))
+    if src.dtype != torch.float32 or dst.dtype != torch.float32:
+        raise ValueError("images are float32")
+    loss = torch.empty(nInst, dtype=torch.float64, device=src.device)
+    gradA = torch.empty((nInst, nC, 6), dtype=torch.float64, device=src.device)
+    ctx = context_for(src)
+    ctx.call("cpab_ssd_grad2d", tess, nC, nInst, As.contiguous().data_ptr(), src.contiguous().data_ptr(),
+             dst.contiguous().data_ptr(), int(H), int(W), dt, nT, nS, nC0, nC1, ix, iy,
+             pos_out.data_ptr() if pos_out is not None else None, loss.data_ptr(), gradA.data_ptr())
+    return loss, gradA
+
+
+def _function_classes():
+    torch = require_gpu()
+
+    class WarpSSD(torch.autograd.Function):
+        """loss_i = 0.5 sum_x (src_i(T_{theta_i}(x)) - dst_i(x))^2 for a batch
+        of image pairs on the pixel grid; theta (nInst, d), B (D, d)."""
+
+        @staticmethod
+        def forward(fctx, theta, B, src, dst, space, params, sign):
+            As = _signed_tables(torch, theta.detach(), B, sign, space.nC)
+            loss, gradA = ssd_loss_and_grad(space, As, src, dst, params)
+            fctx.save_for_backward(gradA, B)
+            fctx.sign = float(sign)
+            return loss
+
+        @staticmethod
+        def backward(fctx, gl):
+            gradA, B = fctx.saved_tensors
+            g = (gl[:, None] * gradA.reshape(gradA.shape[0], -1)) @ B
+            return fctx.sign * g, None, None, None, None, None, None
+
+    class TransformSimple(torch.autograd.Function):
+        """T_theta(pts) by calc_T_simple; its backward is the adjoint kernel.
+        theta (d,), pts (N, 2), B (D, d)."""
+
+        @staticmethod
+        def forward(fctx, theta, pts, B, space, params, sign):
+            As = _signed_tables(torch, theta.detach().reshape(1, -1), B, sign, space.nC)
+            out = torch.empty_like(pts)
+            tess, nC, nC0, nC1, ix, iy = _geom2d(space)
+            dt, nT, nS = _params(params)
+            context_for(pts).call("cpab_calc_T_simple", 2, tess, nC, pts.contiguous().data_ptr(), out.data_ptr(),
+                                  As.data_ptr(), dt, nT, nS, int(pts.shape[0]), nC0, nC1, 0, ix, iy, 1.0)
+            fctx.save_for_backward(pts, As, B)
+            fctx.space, fctx.params, fctx.sign = space, params, float(sign)
+            return out
+
+        @staticmethod
+        def backward(fctx, gout):
+            pts, As, B = fctx.saved_tensors
+            gradA = calc_T_simple_vjp(fctx.space, pts, gout.contiguous(), As[0], fctx.params)
+            g = gradA.reshape(1, -1) @ B
+            return fctx.sign * g.reshape(-1), None, None, None, None, None
+
+    return WarpSSD, TransformSimple
+
+
+_FN = None
+
+
+def warp_ssd(theta, B, src, dst, space, params, sign=-1):
+    """Differentiable batch SSD loss (see WarpSSD); sign -1 warps with the
+    inverse transform, as remap_fwd does with calc_T_inv."""
+    global _FN
+    if _FN is None:
+        _FN = _function_classes()
+    return _FN[0].apply(theta, B, src, dst, space, params, sign)
+
+
+def transform_simple(theta, pts, B, space, params, sign=1):
+    """Differentiable calc_T_simple of pts (see TransformSimple)."""
+    global _FN
+    if _FN is None:
+        _FN = _function_classes()
+    return _FN[1].apply(theta, pts, B, space, params, sign)
+
+
+def fd_directional(loss_fn, theta, v, h=1e-6):
+    """Central finite difference of loss_fn along v (FP64)."""
+    return (loss_fn(theta + h * v) - loss_fn(theta - h * v)) / (2.0 * h)
+
+
+class PairsProblem:
+    """A batch of image pairs for gradient-based registration: nInst FP32
+    blob images (n x n), each target = its source warped by T^{-1} of a prior
+    draw theta*_i (calc_T_simple on the pixel grid, FP64 bilinear sample)
+    plus N(0, noise^2) pixel noise; the 4 x 4 x 4-triangle tessellation of
+    [0, n]^2 with zero normal boundary velocity."""
+
+    def __init__(self, n=128, nInst=64, seed=0, nCs=(4, 4), noise=0.02, n_blobs=40, nTimeSteps=100, nStepsODE=10,
+                 t=1.0, length_scale=0.1, variance=1.0):
+        import numpy as np
+        from . import configs
+        from .registration import pixel_basis
+        torch = require_gpu()
+        self.n, self.nInst = n, nInst
+        self.space_unit = configs.unit_space(nCs, "I")
+        self.space = configs.pixel_space(n, nCs, "I")
+        self.cov = self.space_unit.se_prior_cov(length_scale, variance)
+        self.B = pixel_basis(self.space_unit, n)
+        self.d = self.space_unit.d
+        self.params = dict(dt=t / nTimeSteps, nTimeSteps=nTimeSteps, nStepsODEsolver=nStepsODE)
+        rng = np.random.default_rng(seed)
+        self.src_np = np.stack([configs.blob_image(n, n_blobs, seed=seed * 100003 + i) for i in range(nInst)])
+        self.theta_star = np.stack([self.space_unit.sample_theta(self.cov, rng) for _ in range(nInst)])
+        self.Bt = torch.from_numpy(np.ascontiguousarray(self.B)).cuda()
+        self.src = torch.from_numpy(self.src_np).cuda()
+        # targets: src_i(T_i^{-1}(x)) + noise, warped in FP64 by the same kernel
+        As = _signed_tables(torch, torch.from_numpy(self.theta_star).cuda(), self.Bt, -1, self.space.nC)
+        zero = torch.zeros_like(self.src)
+        pos = torch.empty((nInst, n * n, 2), dtype=torch.float64, device="cuda")
+        ssd_loss_and_grad(self.space, As, self.src, zero, self.params, pos_out=pos)
+        warped = torch.empty_like(self.src)
+        ctx = context_for(self.src)
+        for i in range(nInst):
+            ctx.call("cpab_remap2d", pos[i].data_ptr(), self.src[i].data_ptr(), warped[i].data_ptr(), n, n)
+        noise_t = torch.from_numpy(rng.normal(0, noise, (nInst, n, n)).astype(np.float32)).cuda()
+        self.dst = (warped + noise_t).contiguous()
+
+    def loss(self, theta):
+        return warp_ssd(theta, self.Bt, self.src, self.dst, self.space, self.params, sign=-1)
+
+
+def gradient_descent(problem, steps, lr, theta0=None):
+    """Plain gradient descent on every pair's theta (torch.optim.SGD over the
+    WarpSSD Function).  Returns (theta, loss history (steps, nInst))."""
+    torch = require_gpu()
+    theta = (torch.zeros((problem.nInst, problem.d), dtype=torch.float64, device="cuda") if theta0 is None
+             else theta0.clone()).requires_grad_(True)
+    opt = torch.optim.SGD([theta], lr=lr)
+    hist = []
+    for _ in range(steps):
+        opt.zero_grad()
+        loss = problem.loss(theta)
+        loss.sum().backward()
+        opt.step()
+        hist.append(loss.detach())
+    return theta.detach(), torch.stack(hist)

@@ -14,6 +14,7 @@
 #include "../../include/cpab_hip.h"
 #include "cpab_kernels.h"
 #include "cpab_mcmc.h"
+#include "cpab_adjoint.h"
 
 #pragma clang fp contract(off)
 
@@ -33,6 +34,8 @@ struct cpab_ctx {
   double prof_ms = 0.0;
   double* scratch = nullptr;  // reduction partials
   size_t scratch_bytes = 0;
+  double* ws = nullptr;       // adjoint checkpoints and partials
+  size_t ws_bytes = 0;
 };
 
 namespace {
@@ -289,6 +292,7 @@ int cpab_destroy(cpab_ctx* ctx) {
   for (auto& p : ctx->ev) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
   for (auto& p : ctx->ev_free) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
   if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->ws) (void)hipFree(ctx->ws);
   delete ctx;
   return CPAB_OK;
 }
@@ -546,6 +550,125 @@ int cpab_ssd_ll(cpab_ctx* ctx, const float* x, const float* y, int64_t n, double
   hipLaunchKernelGGL(k_reduce_ll, dim3(1), dim3(64), 0, ctx->stream, ctx->scratch, nb, sigma, ll);
   HIP_OK(ctx, hipGetLastError());
   return CPAB_OK;
+}
+
+// ------------------------------------------------------------- adjoint
+}  // extern "C"
+
+namespace {
+
+int ensure_ws(cpab_ctx* ctx, size_t bytes) {
+  if (ctx->ws_bytes >= bytes) return CPAB_OK;
+  if (ctx->ws) {
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_OK(ctx, hipFree(ctx->ws));
+    ctx->ws = nullptr;
+    ctx->ws_bytes = 0;
+  }
+  hipError_t e = hipMalloc(&ctx->ws, bytes);
+  if (e != hipSuccess) return fail(ctx, CPAB_ERR_NOMEM, "hipMalloc(%zu) for the adjoint workspace: %s", bytes,
+                                   hipGetErrorString(e));
+  ctx->ws_bytes = bytes;
+  return CPAB_OK;
+}
+
+// Shared by the two adjoint entries: checks, workspace, launch, reduction.
+int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts, int H, int W, const double* pos0,
+              const double* lam, const float* src, const float* dst, const double* As, double dt, int nT, int nS,
+              int nC0, int nC1, double ix, double iy, double* pos_out, double* loss, double* gradA) {
+  Geom g;
+  bool pow2;
+  int r = make_geom(ctx, 2, tess, nC0, nC1, 0, ix, iy, 1.0, &g, &pow2);
+  if (r) return r;
+  if ((r = check_nC(ctx, 2, tess, nC, nC0, nC1, 0))) return r;
+  if (nInst < 1 || nInst > 65535) return fail(ctx, CPAB_ERR_ARG, "nInst must be in [1, 65535]");
+  if (nPts < 0 || nT < 0 || nS < 0) return fail(ctx, CPAB_ERR_ARG, "bad sizes");
+  if (nS > ADJ_NSMAX) return fail(ctx, CPAB_ERR_UNSUPPORTED, "nStepsODE > %d is not supported by the adjoint", ADJ_NSMAX);
+  if (!As || !gradA) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  const int NT = nC * 6;
+  const size_t lds = (size_t)2 * NT * sizeof(double);
+  if (lds > (size_t)ctx->max_lds) return fail(ctx, CPAB_ERR_UNSUPPORTED, "tables exceed LDS");
+  if (nPts == 0 || nT == 0) {  // the identity map: zero gradient (and loss of the unwarped images in MODE 1)
+    if (mode == 1) return fail(ctx, CPAB_ERR_UNSUPPORTED, "empty integration in the SSD adjoint");
+    HIP_OK(ctx, hipMemsetAsync(gradA, 0, sizeof(double) * NT * nInst, ctx->stream));
+    if (pos_out && nPts > 0)
+      HIP_OK(ctx, hipMemcpyAsync(pos_out, pos0, sizeof(double) * 2 * nPts * nInst, hipMemcpyDeviceToDevice,
+                                 ctx->stream));
+    return CPAB_OK;
+  }
+  const int64_t nbx = (nPts + 255) / 256;
+  if (nbx > 0x7fffffffLL) return fail(ctx, CPAB_ERR_ARG, "too many points");
+  const size_t ck = (size_t)nInst * nT * nPts * 2, pa = (size_t)nInst * nbx * (NT + 1);
+  if ((r = ensure_ws(ctx, (ck + pa) * sizeof(double)))) return r;
+  AdjDesc D{};
+  D.nInst = nInst;
+  D.nC = nC;
+  D.nPts = nPts;
+  D.H = H;
+  D.W = W;
+  D.As = As;
+  D.pos0 = pos0;
+  D.lam = lam;
+  D.pos_out = pos_out;
+  D.src = src;
+  D.dst = dst;
+  D.ckpt = ctx->ws;
+  D.partial = ctx->ws + ck;
+  D.h = dt / (double)nS;
+  D.nT = nT;
+  D.nS = nS;
+  D.g = g;
+  const dim3 grid((unsigned)nbx, (unsigned)nInst);
+  hipEvent_t a = nullptr, b = nullptr;
+  if ((r = prof_begin(ctx, &a, &b))) return r;
+#define ADJ_LAUNCH(T_, P_)                                                                          \
+  do {                                                                                              \
+    if (mode == 0) {                                                                                \
+      auto k = k_adjoint2d<T_, P_, 0>;                                                              \
+      allow_lds(k, lds);                                                                            \
+      hipLaunchKernelGGL(k, grid, dim3(256), lds, ctx->stream, D);                                  \
+    } else {                                                                                        \
+      auto k = k_adjoint2d<T_, P_, 1>;                                                              \
+      allow_lds(k, lds);                                                                            \
+      hipLaunchKernelGGL(k, grid, dim3(256), lds, ctx->stream, D);                                  \
+    }                                                                                               \
+  } while (0)
+  if (tess == TESS_I) {
+    if (pow2) ADJ_LAUNCH(TESS_I, true);
+    else ADJ_LAUNCH(TESS_I, false);
+  } else {
+    if (pow2) ADJ_LAUNCH(TESS_II, true);
+    else ADJ_LAUNCH(TESS_II, false);
+  }
+#undef ADJ_LAUNCH
+  HIP_OK(ctx, hipGetLastError());
+  if ((r = prof_end(ctx, a, b))) return r;
+  hipLaunchKernelGGL(k_adj_reduce, dim3(nInst), dim3(256), 0, ctx->stream, D.partial, (int)nbx, NT, gradA, loss);
+  HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpab_calc_T_simple_vjp(cpab_ctx* ctx, int tess, int nC, int nInst, const double* pos0, const double* lam,
+                           const double* As, double dt, int nTimeSteps, int nStepsODE, int64_t nPts, int nC0, int nC1,
+                           double inc_x, double inc_y, double* pos_out, double* gradA) {
+  if (!ctx) return CPAB_ERR_ARG;
+  if (nPts > 0 && (!pos0 || !lam)) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  return adjoint2d(ctx, 0, tess, nC, nInst, nPts, 0, 0, pos0, lam, nullptr, nullptr, As, dt, nTimeSteps, nStepsODE,
+                   nC0, nC1, inc_x, inc_y, pos_out, nullptr, gradA);
+}
+
+int cpab_ssd_grad2d(cpab_ctx* ctx, int tess, int nC, int nInst, const double* As, const float* src, const float* dst,
+                    int H, int W, double dt, int nTimeSteps, int nStepsODE, int nC0, int nC1, double inc_x,
+                    double inc_y, double* pos_out, double* loss, double* gradA) {
+  if (!ctx) return CPAB_ERR_ARG;
+  if (H < 1 || W < 1) return fail(ctx, CPAB_ERR_ARG, "images must be non-empty");
+  if (!src || !dst || !loss) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  return adjoint2d(ctx, 1, tess, nC, nInst, (int64_t)H * W, H, W, nullptr, nullptr, src, dst, As, dt, nTimeSteps,
+                   nStepsODE, nC0, nC1, inc_x, inc_y, pos_out, loss, gradA);
 }
 
 // ------------------------------------------------------------- MCMC

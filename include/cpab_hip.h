@@ -103,6 +103,30 @@ int cpab_calc_grad_theta(cpab_ctx *ctx, int dim, int tess, int nC, const double 
                          int nTimeSteps, int nStepsODE, int64_t nPts, int nC0, int nC1, int nC2,
                          double inc_x, double inc_y, double inc_z);
 
+/* Reverse-mode counterpart of calc_grad_theta for 2D (the same derivative of
+ * calc_T_simple's RK2 map, cpab/gpu/calc_2Dto2D.cu:202-319, 412-445, taken
+ * backwards): for nInst instances of nPts points each, given the
+ * cotangents lam (nInst, nPts, 2) of the end points, gradA (nInst, nC, 6) =
+ * sum_x lam(x)^T dT(x)/dA, A the integrated (signed) tables As (nInst, nC, 6).
+ * dtheta follows as B^T gradA (times the sign).  One checkpoint per time step
+ * in a library workspace (16 * nTimeSteps bytes per point); the substeps are
+ * recomputed.  nStepsODE <= 16.  pos_out (optional): the end points. */
+int cpab_calc_T_simple_vjp(cpab_ctx *ctx, int tess, int nC, int nInst, const double *pos0,
+                           const double *lam, const double *As, double dt, int nTimeSteps,
+                           int nStepsODE, int64_t nPts, int nC0, int nC1, double inc_x,
+                           double inc_y, double *pos_out, double *gradA);
+
+/* The SSD loss of images warped by calc_T_simple and its adjoint, fused, for
+ * nInst image pairs: the points are the H x W pixel grid (x = column,
+ * y = row), w(x) = src(T(x)) sampled bilinearly in FP64 (the cpab_remap2d
+ * convention), loss[i] = 0.5 sum_x (w(x) - dst(x))^2 and gradA[i] = dloss/dA
+ * (the loss of _ScaleDependentLogLikelihoodGaussian.py:71-109 up to the
+ * factor -1/sigma^2).  src, dst: (nInst, H, W) float32. */
+int cpab_ssd_grad2d(cpab_ctx *ctx, int tess, int nC, int nInst, const double *As, const float *src,
+                    const float *dst, int H, int W, double dt, int nTimeSteps, int nStepsODE,
+                    int nC0, int nC1, double inc_x, double inc_y, double *pos_out, double *loss,
+                    double *gradA);
+
 /* calc_v: Calcs.calc_velocities (cpab/gpu/Calcs.py:816-924), kernel
  * cpab/gpu/calc_2Dto2D.cu:561-602.  As is the unsigned table. */
 int cpab_calc_v(cpab_ctx *ctx, int dim, int tess, int nC, const double *pts, double *vel,

@@ -211,3 +211,36 @@ def transform(geom, nC, As, sign, pos0, dt, nTimeSteps, nStepsODE, nthreads=0):
 
 def max_threads():
     return lib().oracle_max_threads()
+
+
+def bilinear64(img, pts):
+    """The remap convention (OpenCV INTER_LINEAR, zero outside, a full pixel
+    or more outside -> 0) evaluated in float64, with its gradient: returns
+    (w, dw/dx, dw/dy) at pts (N, 2) = (x = column, y = row).  This is the
+    interpolant the SSD adjoint differentiates (no reference counterpart: the
+    reference's remap is the un-vendored of.gpu.remap)."""
+    img = np.asarray(img, dtype=np.float64)
+    H, W = img.shape
+    x = np.asarray(pts[:, 0], dtype=np.float64)
+    y = np.asarray(pts[:, 1], dtype=np.float64)
+    with np.errstate(invalid="ignore"):
+        xf, yf = np.floor(x), np.floor(y)
+        ok = (xf >= -1) & (xf < W) & (yf >= -1) & (yf < H)
+    xf = np.where(ok, xf, 0.0)
+    yf = np.where(ok, yf, 0.0)
+    fx, fy = np.where(ok, x - xf, 0.0), np.where(ok, y - yf, 0.0)
+    x0, y0 = xf.astype(np.int64), yf.astype(np.int64)
+
+    def px(yy, xx):
+        inside = ok & (xx >= 0) & (xx < W) & (yy >= 0) & (yy < H)
+        return np.where(inside, img[np.clip(yy, 0, H - 1), np.clip(xx, 0, W - 1)], 0.0)
+
+    v00, v01 = px(y0, x0), px(y0, x0 + 1)
+    v10, v11 = px(y0 + 1, x0), px(y0 + 1, x0 + 1)
+    gx, gy = 1.0 - fx, 1.0 - fy
+    a = gx * v00 + fx * v01
+    b = gx * v10 + fx * v11
+    w = np.where(ok, gy * a + fy * b, 0.0)
+    dwdx = np.where(ok, gy * (v01 - v00) + fy * (v11 - v10), 0.0)
+    dwdy = np.where(ok, b - a, 0.0)
+    return w, dwdx, dwdy

@@ -64,3 +64,34 @@ def test_ssd_closed_form(orc):
     y = np.full(n, 0.25, np.float32)
     assert orc.ssd_ll(x, y, 0.02) == -0.5 * n * 0.25 / 0.0004
     assert orc.ssd_ll(x, x, 0.02) == 0.0
+
+
+def test_bilinear64_exact_on_affine_images_and_matches_scipy(orc):
+    """The FP64 bilinear interpolant and gradient the SSD adjoint uses: exact
+    on an affine image (values and gradient) away from the border, equal to
+    scipy's order-1 map_coordinates inside, zero a full pixel outside."""
+    import numpy as np
+    from scipy.ndimage import map_coordinates
+    H, W = 20, 30
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float64)
+    img = 0.5 + 0.03 * xx - 0.02 * yy
+    rng = np.random.default_rng(4)
+    pts = np.stack([rng.uniform(0, W - 1, 500), rng.uniform(0, H - 1, 500)], axis=1)
+    w, gx, gy = orc.bilinear64(img, pts)
+    np.testing.assert_allclose(w, 0.5 + 0.03 * pts[:, 0] - 0.02 * pts[:, 1], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(gx, 0.03, atol=1e-14)
+    np.testing.assert_allclose(gy, -0.02, atol=1e-14)
+    img2 = rng.random((H, W))
+    w2, _, _ = orc.bilinear64(img2, pts)
+    ref = map_coordinates(img2, [pts[:, 1], pts[:, 0]], order=1, mode="constant", cval=0.0)
+    np.testing.assert_allclose(w2, ref, atol=1e-13)
+    far = np.array([[-1.5, 3.0], [W + 0.5, 3.0], [3.0, H], [np.nan, 1.0]])
+    w3, g3x, g3y = orc.bilinear64(img2, far)
+    assert (w3 == 0).all() and (g3x == 0).all() and (g3y == 0).all()
+    # gradient = central differences of the interpolant away from pixel edges
+    q = np.array([[3.3, 4.6], [10.25, 7.75]])
+    e = 1e-7
+    for k in range(2):
+        d = np.zeros(2); d[k] = e
+        num = (orc.bilinear64(img2, q + d)[0] - orc.bilinear64(img2, q - d)[0]) / (2 * e)
+        np.testing.assert_allclose(orc.bilinear64(img2, q)[1 + k], num, rtol=1e-7)
