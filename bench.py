@@ -118,6 +118,97 @@ def config1_extra(torch, ctx):
             "frac_fp64_peak": flops / (kms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
 
 
+def config3_extra(torch, ctx):
+    """Batching: (a) 1 024 heterogeneous 2D instances (4x4, 8x8 or 16x16 rectangles
+    x 4 triangles, own theta, N ~ logU[1e3, 1e6] points U[0,1]^2) in one
+    persistent graph-captured launch (+ one prep launch), schedule autotuned,
+    against one update_pat + calc_T call per instance; (b) 256 registration
+    chains, all proposals of a step in one launch, against the host-driven
+    one-proposal-at-a-time loop."""
+    import numpy as np
+    from cpabdiffeo_amd import batch
+    from cpabdiffeo_amd.registration import DeviceMetropolis, HostMetropolis, RegistrationProblem
+    spaces, insts = batch.heterogeneous_instances(1024, seed=0)
+    plan = batch.BatchPlan(2, "I", [((g, g), (1.0 / g, 1.0 / g), n) for g, _, n in insts])
+    As = torch.from_numpy(np.concatenate([Av.reshape(-1, 6) for _, Av, _ in insts])).cuda()
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0)
+    pts = torch.rand((plan.nPts, 2), dtype=torch.float64, device="cuda", generator=gen)
+    out = torch.empty_like(pts)
+    params = dict(dt=0.01, nTimeSteps=100, nStepsODEsolver=10)
+    tables = plan.prep(As, 1, params["dt"])
+    tune = plan.autotune(tables, pts, out, params["dt"], 100, 10)
+
+    def batched():
+        plan.prep(As, 1, params["dt"], tables)
+        plan.calc_T(tables, pts, out, params["dt"], 100, 10, use_graph=True)
+
+    batched()
+    t_b = min(batch.timed(batched) for _ in range(3))
+    ctx.prof_enable(True)
+    ctx.prof_read()
+    plan.calc_T(tables, pts, out, params["dt"], 100, 10, use_graph=True)
+    cnt, ms = ctx.prof_read()
+    ctx.prof_enable(False)
+    out2 = torch.empty_like(pts)
+    batch.plain_per_instance(spaces, insts, pts, out2, params)
+    t_p = batch.timed(lambda: batch.plain_per_instance(spaces, insts, pts, out2, params))
+    same = bool(torch.equal(out, out2))
+    a = {"workload": "1024 instances, tess {4,8,16}^2 x 4 triangles, N ~ logU[1e3,1e6] (total %d points), "
+                     "calc_T_fwd t=1 100x10 fp64" % plan.nPts,
+         "points": plan.nPts, "batched_ms": t_b * 1e3, "batched_points_per_sec": plan.nPts / t_b,
+         "batched_kernel_ms": ms / cnt, "schedule": list(plan.schedule.key()),
+         "autotune_ms": {"%d/%d/%d" % k: v for k, v in tune.items()},
+         "plain_ms": t_p * 1e3, "plain_points_per_sec": plan.nPts / t_p, "speedup": t_p / t_b,
+         "bit_identical_to_plain": same}
+    plan.close()
+    del pts, out, out2
+    torch.cuda.empty_cache()
+    prob = RegistrationProblem(n=256, seed=0)
+    mh = DeviceMetropolis(prob, nChains=256, scale=0.01, seed=0, spec_depth=1)
+    mh.set_theta(np.zeros(mh.d))
+    mh.run(16)
+    torch.cuda.synchronize()
+    n_dev = 48
+    t_dev = batch.timed(lambda: mh.run(n_dev))
+    mh.close()
+    hm = HostMetropolis(prob, scale=0.01, seed=0)
+    hm.set_theta(np.zeros(hm.d))
+    hm.run(8)
+    n_host = 100
+    t_host = batch.timed(lambda: hm.run(n_host))
+    b = {"workload": "256 chains of the 256x256 registration, one step (256 proposals) per launch",
+         "proposals_per_sec": 256 * n_dev / t_dev, "plain_proposals_per_sec": n_host / t_host,
+         "speedup": (256 * n_dev / t_dev) / (n_host / t_host)}
+    return {"instances_1024": a, "chains_256": b}
+
+
+def config5_extra(torch, n_req=2000, rate=2000.0):
+    """Serving: Poisson arrivals at 2 000 req/s of the 1D / 2D / 3D mix, HBM
+    budget 64 GB, continuous batching; then the same request set all at once
+    (capacity) and a sample of it one request at a time through the
+    single-call path."""
+    from cpabdiffeo_amd import serving
+    kinds = serving.default_kinds()
+    srv = serving.Server(kinds, budget_bytes=64 << 30)
+    srv.run(serving.make_mix(100, kinds, rate=rate, seed=99))  # warm the allocator and clocks
+    st = srv.run(serving.make_mix(n_req, kinds, rate=rate, seed=0))
+    cap = srv.run(serving.make_mix(n_req, kinds, rate=rate, seed=0), realtime=False)
+    sample = serving.make_mix(n_req, kinds, rate=rate, seed=0)[:300]
+    t0 = time.perf_counter()
+    for r in sample:
+        srv.alone(r)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    plain_pps = sum(r.n for r in sample) / t_plain
+    st.update({"workload": "%d requests, Poisson %g req/s: 40%% 1D (50 cells, 1e4-1e6 pts), 40%% 2D grids 64^2-1024^2 "
+                           "(4x4 or 8x8 x4 triangles), 20%% 3D 64^3 volumes (4x4x4 x5 tets, calc_T_inv + trilinear "
+                           "remap); t=1 100x10 fp64; HBM budget 64 GB" % (n_req, rate),
+               "capacity_points_per_sec": cap["points_per_sec"], "capacity_requests_per_sec": cap["requests_per_sec"],
+               "plain_points_per_sec": plain_pps, "capacity_vs_plain": cap["points_per_sec"] / plain_pps})
+    return st
+
+
 def config4_extra(torch, ctx, steps=200, lr=3e-4):
     """Gradient descent on 64 pairs of 128x128 blob images (one theta per pair),
     gradient by the fused SSD adjoint; plus a central-difference check."""
@@ -140,23 +231,40 @@ def config4_extra(torch, ctx, steps=200, lr=3e-4):
     sub = p["nTimeSteps"] * p["nStepsODEsolver"]
     flops = npts * sub * (2 * F_SUB + F_ADJ_SUB)
     kus = ms / cnt * 1e3
-    # central differences (h = 1e-6, FP64 loss) along one random direction, every pair
-    rng = np.random.default_rng(0)
-    v = torch.from_numpy(rng.standard_normal(theta.shape)).cuda()
+    # (1) the reference's own derivative: the forward sensitivity (solveODE2) of
+    # pair 0 on the device, contracted with the SSD cotangent, vs the adjoint
+    sp, B = prob.space, prob.B
+    th0 = theta[0]
+    As0 = (-(th0 @ prob.Bt.t().contiguous())).reshape(sp.nC, 6).contiguous()
+    grid = torch.from_numpy(sp.tessellation.dense_grid_img()).cuda()
+    end = torch.empty_like(grid)
+    Bs = torch.from_numpy(-np.ascontiguousarray(B.T.reshape(prob.d, sp.nC, 2, 3))).cuda()
+    sens = torch.empty((grid.shape[0], 2, prob.d), dtype=torch.float64, device="cuda")
+    sp._gpu_calcs.calc_grad_theta(None, None, As0, Bs, grid, p["dt"], p["nTimeSteps"], p["nStepsODEsolver"], end,
+                                  sens, 2, 2, sp.nCs, sp.incs)
+    _, lam = adjoint.ssd_cotangent(prob.src[0], prob.dst[0], end)
+    g_ref = torch.einsum("ni,nij->j", lam, sens)
     th = theta.clone().requires_grad_(True)
     loss = prob.loss(th)
     loss.sum().backward()
+    g_adj = th.grad[0]
+    ref_err = ((g_adj - g_ref).abs().max() / g_ref.abs().max()).item()
+    # (2) central differences (h = 1e-6, FP64 loss) along one random direction, every pair
+    rng = np.random.default_rng(0)
+    v = torch.from_numpy(rng.standard_normal(theta.shape)).cuda()
     an = (th.grad * v).sum(1)
     with torch.no_grad():
         fd = (prob.loss(theta + 1e-6 * v) - prob.loss(theta - 1e-6 * v)) / 2e-6
-    rel = ((fd - an).abs() / an.abs().clamp_min(1e-12)).max().item()
+    rel = ((fd - an).abs() / an.abs().clamp_min(1e-12)).cpu().numpy()
     return {"workload": "64 pairs of 128x128 fp32 blob images, nC=64 tess I, one theta per pair, %d gradient-descent "
                         "steps (lr %g), calc_T_simple t=1 100x10 fp64, SSD adjoint" % (steps, lr),
             "ms_per_step": el / steps * 1e3, "steps_per_sec": steps / el, "point_gradients_per_sec": npts * steps / el,
             "adjoint_kernel_us": kus, "adjoint_alg_tflops": flops / (kus * 1e-6) / 1e12,
             "adjoint_frac_fp64_peak": flops / (kus * 1e-6) / 1e12 / FP64_PEAK_TFLOPS,
             "loss_mean_first": float(h[0].mean()), "loss_mean_last": float(h[-1].mean()),
-            "fd_max_rel_err": rel}
+            "reference_gradient_max_rel_err": ref_err,
+            "fd_rel_err_median": float(np.median(rel)), "fd_rel_err_max": float(rel.max()),
+            "fd_pairs_within_1e-4": float((rel < 1e-4).mean())}
 
 
 def main():
@@ -250,12 +358,16 @@ def main():
     flops_per_launch = (st[0] * npts * (100 * F_FAST + F_REMAP_SSD) + st[1] * 10 * F_SUB) / launches
     achieved = flops_per_launch / kern_s / 1e12
 
-    extra = gd = None
+    extra = gd = batching = serve = None
     if not args.no_extra:
         log("bench: config-1 calc_T_fwd extra")
         extra = config1_extra(torch, ctx)
+        log("bench: config-3 batching extra")
+        batching = config3_extra(torch, ctx)
         log("bench: config-4 gradient descent extra")
         gd = config4_extra(torch, ctx)
+        log("bench: config-5 serving extra")
+        serve = config5_extra(torch)
 
     cpu = None
     if not args.no_cpu:
@@ -308,6 +420,8 @@ def main():
         "cpu_baseline": cpu,
         "calc_T_fwd_512x512": extra,
         "gradient_descent_64x128x128": gd,
+        "batching": batching,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
 

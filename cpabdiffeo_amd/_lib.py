@@ -33,6 +33,12 @@ SIGNATURES = {
     "cpab_calc_grad_theta": [_p, _i, _i, _i, _p, _p, _p, _p, _p, _i, _d, _i, _i, _i64] + _GEO,
     "cpab_calc_T_simple_vjp": [_p, _i, _i, _i, _p, _p, _p, _d, _i, _i, _i64, _i, _i, _d, _d, _p, _p],
     "cpab_ssd_grad2d": [_p, _i, _i, _i, _p, _p, _p, _i, _i, _d, _i, _i, _i, _i, _d, _d, _p, _p, _p],
+    "cpab_batch_create": [_p, _i, _i, _i, _p, ctypes.POINTER(_p)],
+    "cpab_batch_sizes": [_p, _p, _p, _p, _p],
+    "cpab_batch_prep": [_p, _p, _i, _d, _p],
+    "cpab_batch_calc_T": [_p, _p, _p, _p, _d, _i, _i, _p, _i, _p],
+    "cpab_batch_destroy": [_p],
+    "cpab_calc_T_items": [_p, _i, _i, _i, _p, _d, _i, _i, _p],
     "cpab_calc_v": [_p, _i, _i, _i, _p, _p, _p, _i64] + _GEO,
     "cpab_calc_cell_idx": [_p, _i, _i, _p, _p, _i64] + _GEO,
     "cpab_remap2d": [_p, _p, _p, _p, _i, _i],

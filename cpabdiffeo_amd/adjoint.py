@@ -147,6 +147,35 @@ def fd_directional(loss_fn, theta, v, h=1e-6):
     return (loss_fn(theta + h * v) - loss_fn(theta - h * v)) / (2.0 * h)
 
 
+def ssd_cotangent(src, dst, pts):
+    """Torch FP64 restatement, for self-checks, of what the SSD kernel
+    differentiates: w = the bilinear sample of src (H, W) at pts (N, 2)
+    (cpab_remap2d convention, zero a full pixel or more outside) and the
+    cotangent (w - dst) * grad w of 0.5 (w - dst)^2."""
+    torch = require_gpu()
+    H, W = src.shape
+    img = src.double()
+    x, y = pts[:, 0], pts[:, 1]
+    xf, yf = torch.floor(x), torch.floor(y)
+    ok = (xf >= -1) & (xf < W) & (yf >= -1) & (yf < H)
+    fx, fy = x - xf, y - yf
+    x0, y0 = xf.long(), yf.long()
+
+    def px(yy, xx):
+        inside = ok & (xx >= 0) & (xx < W) & (yy >= 0) & (yy < H)
+        return torch.where(inside, img[yy.clamp(0, H - 1), xx.clamp(0, W - 1)], torch.zeros_like(x))
+
+    v00, v01, v10, v11 = px(y0, x0), px(y0, x0 + 1), px(y0 + 1, x0), px(y0 + 1, x0 + 1)
+    a = (1 - fx) * v00 + fx * v01
+    b = (1 - fx) * v10 + fx * v11
+    zero = torch.zeros_like(x)
+    w = torch.where(ok, (1 - fy) * a + fy * b, zero)
+    gx = torch.where(ok, (1 - fy) * (v01 - v00) + fy * (v11 - v10), zero)
+    gy = torch.where(ok, b - a, zero)
+    r = w - dst.reshape(-1).double()
+    return w, torch.stack([r * gx, r * gy], dim=1)
+
+
 class PairsProblem:
     """A batch of image pairs for gradient-based registration: nInst FP32
     blob images (n x n), each target = its source warped by T^{-1} of a prior

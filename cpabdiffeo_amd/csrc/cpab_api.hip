@@ -15,6 +15,7 @@
 #include "cpab_kernels.h"
 #include "cpab_mcmc.h"
 #include "cpab_adjoint.h"
+#include "cpab_batch.h"
 
 #pragma clang fp contract(off)
 
@@ -36,9 +37,16 @@ struct cpab_ctx {
   size_t scratch_bytes = 0;
   double* ws = nullptr;       // adjoint checkpoints and partials
   size_t ws_bytes = 0;
+  WorkItem* items = nullptr;  // work items of cpab_calc_T_items
+  size_t items_cap = 0;
+  unsigned int* queue = nullptr;
+  std::vector<WorkItem> h_items;
 };
 
 namespace {
+
+// The substep length of calc_T: dt / nStepsODE (any positive count).
+double dtdiv(double dt, int nS) { return dt / (double)(nS > 0 ? nS : 1); }
 
 int fail(cpab_ctx* ctx, int code, const char* fmt, ...) {
   char buf[512];
@@ -293,6 +301,8 @@ int cpab_destroy(cpab_ctx* ctx) {
   for (auto& p : ctx->ev_free) { (void)hipEventDestroy(p.first); (void)hipEventDestroy(p.second); }
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->items) (void)hipFree(ctx->items);
+  if (ctx->queue) (void)hipFree(ctx->queue);
   delete ctx;
   return CPAB_OK;
 }
@@ -669,6 +679,362 @@ int cpab_ssd_grad2d(cpab_ctx* ctx, int tess, int nC, int nInst, const double* As
   if (!src || !dst || !loss) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
   return adjoint2d(ctx, 1, tess, nC, nInst, (int64_t)H * W, H, W, nullptr, nullptr, src, dst, As, dt, nTimeSteps,
                    nStepsODE, nC0, nC1, inc_x, inc_y, pos_out, loss, gradA);
+}
+
+// ------------------------------------------------------------- batch
+}  // extern "C"
+
+struct cpab_batch {
+  cpab_ctx* ctx = nullptr;
+  int dim = 2, tess = TESS_I, nInst = 0, maxNC = 0;
+  bool pow2 = true;
+  int64_t nCells = 0, nPts = 0, tableDoubles = 0;
+  std::vector<BatchInst> h;
+  BatchInst* d_inst = nullptr;
+  unsigned int* queue = nullptr;
+  BatchTile* d_tiles = nullptr;
+  int nTiles = 0, tile = 0;
+  // captured launch (reset + persistent kernel) and the arguments it was captured with
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
+  double key_h = 0.0;
+  int key_i[5] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+void batch_drop_graph(cpab_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+int batch_tiles(cpab_batch* b, int tile) {
+  if (b->d_tiles && b->tile == tile) return CPAB_OK;
+  cpab_ctx* ctx = b->ctx;
+  std::vector<BatchTile> t;
+  for (int i = 0; i < b->nInst; i++) {
+    const BatchInst& I = b->h[i];
+    for (int64_t s = 0; s < I.nPts; s += tile) {
+      BatchTile x{};
+      x.inst = i;
+      x.begin = I.pts + s;
+      x.end = I.pts + (s + tile < I.nPts ? s + tile : I.nPts);
+      t.push_back(x);
+    }
+  }
+  if (t.size() > 0x7fffffff) return fail(ctx, CPAB_ERR_ARG, "too many tiles");
+  HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+  if (b->d_tiles) HIP_OK(ctx, hipFree(b->d_tiles));
+  b->d_tiles = nullptr;
+  b->nTiles = (int)t.size();
+  b->tile = tile;
+  batch_drop_graph(b);
+  if (t.empty()) return CPAB_OK;
+  HIP_OK(ctx, hipMalloc(&b->d_tiles, sizeof(BatchTile) * t.size()));
+  HIP_OK(ctx, hipMemcpy(b->d_tiles, t.data(), sizeof(BatchTile) * t.size(), hipMemcpyHostToDevice));
+  return CPAB_OK;
+}
+
+template <int DIM, int TESS, bool POW2>
+int batch_launch(cpab_batch* b, hipStream_t s, const cpab_batch_sched& q, const double* tables, const double* pos0,
+                 double* out, double h, int nT, int nS, int32_t* nfb) {
+  constexpr int NE = DIM * (DIM + 1);
+  cpab_ctx* ctx = b->ctx;
+  const size_t lds = (size_t)2 * b->maxNC * NE * sizeof(double);
+  auto k = k_integrate_batch<DIM, TESS, POW2>;
+  allow_lds(k, lds);
+  int per_cu = 0;
+  HIP_OK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, q.block, lds));
+  if (per_cu < 1) return fail(ctx, CPAB_ERR_UNSUPPORTED, "the batch kernel does not fit a CU with block %d", q.block);
+  int grid = ctx->n_cu * per_cu;
+  const int64_t chunks = ((int64_t)b->nTiles + q.chunk - 1) / q.chunk;
+  if (grid > chunks) grid = (int)chunks;
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, b->queue);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(q.block), lds, s, b->d_inst, b->d_tiles, b->nTiles, q.chunk, b->queue,
+                     tables, pos0, out, h, nT, nS, nfb);
+  HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+int batch_dispatch(cpab_batch* b, hipStream_t s, const cpab_batch_sched& q, const double* tables,
+                   const double* pos0, double* out, double h, int nT, int nS, int32_t* nfb) {
+#define BATCH_CASE(D_, T_, P_) \
+  if (b->dim == D_ && b->tess == T_ && b->pow2 == P_) return batch_launch<D_, T_, P_>(b, s, q, tables, pos0, out, h, nT, nS, nfb);
+  BATCH_CASE(1, TESS_II, true)
+  BATCH_CASE(1, TESS_II, false)
+  BATCH_CASE(2, TESS_I, true)
+  BATCH_CASE(2, TESS_I, false)
+  BATCH_CASE(2, TESS_II, true)
+  BATCH_CASE(2, TESS_II, false)
+  BATCH_CASE(3, TESS_I, true)
+  BATCH_CASE(3, TESS_I, false)
+  BATCH_CASE(3, TESS_II, true)
+  BATCH_CASE(3, TESS_II, false)
+#undef BATCH_CASE
+  return fail(b->ctx, CPAB_ERR_UNSUPPORTED, "no batch kernel for dim %d tess %d", b->dim, b->tess);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpab_batch_create(cpab_ctx* ctx, int dim, int tess, int nInst, const cpab_batch_inst* insts, cpab_batch** out) {
+  if (!ctx || !insts || !out) return CPAB_ERR_ARG;
+  *out = nullptr;
+  if (nInst < 1) return fail(ctx, CPAB_ERR_ARG, "nInst must be >= 1");
+  const int NE = dim * (dim + 1);
+  std::vector<BatchInst> h(nInst);
+  int64_t cells = 0, pts = 0, tab = 0;
+  int maxNC = 0;
+  bool pow2_all = true, pow2_any = false;
+  for (int i = 0; i < nInst; i++) {
+    const cpab_batch_inst& q = insts[i];
+    bool pow2;
+    BatchInst& I = h[i];
+    std::memset(&I, 0, sizeof I);
+    int r = make_geom(ctx, dim, tess, q.nC0, q.nC1, q.nC2, q.inc_x, q.inc_y, q.inc_z, &I.g, &pow2);
+    if (r) return r;
+    int64_t boxes = q.nC0;
+    if (dim >= 2) boxes *= q.nC1;
+    if (dim >= 3) boxes *= q.nC2;
+    const int64_t nC = boxes * (tess == TESS_II ? 1 : (dim == 2 ? 4 : 5));
+    if ((r = check_nC(ctx, dim, tess, (int)nC, q.nC0, q.nC1, q.nC2))) return r;
+    if (q.nPts < 0) return fail(ctx, CPAB_ERR_ARG, "instance %d: nPts < 0", i);
+    pow2_all = pow2_all && pow2;
+    pow2_any = pow2_any || pow2;
+    I.nC = (int)nC;
+    I.tab = tab;
+    I.pts = pts;
+    I.nPts = q.nPts;
+    I.cell = cells;
+    tab += 2 * nC * NE;
+    pts += q.nPts;
+    cells += nC;
+    if (nC > maxNC) maxNC = (int)nC;
+  }
+  if (pow2_any && !pow2_all)
+    return fail(ctx, CPAB_ERR_UNSUPPORTED,
+                "a batch mixes power-of-two and other increments: make one batch of each");
+  if ((size_t)2 * maxNC * NE * sizeof(double) > (size_t)ctx->max_lds)
+    return fail(ctx, CPAB_ERR_UNSUPPORTED, "the largest tessellation's tables exceed LDS");
+  cpab_batch* b = new (std::nothrow) cpab_batch();
+  if (!b) return CPAB_ERR_NOMEM;
+  b->ctx = ctx;
+  b->dim = dim;
+  b->tess = tess;
+  b->nInst = nInst;
+  b->maxNC = maxNC;
+  b->pow2 = pow2_all;
+  b->nCells = cells;
+  b->nPts = pts;
+  b->tableDoubles = tab;
+  b->h = h;
+  hipError_t e = hipMalloc(&b->d_inst, sizeof(BatchInst) * nInst);
+  if (e == hipSuccess) e = hipMalloc(&b->queue, 256);
+  if (e == hipSuccess) e = hipMemcpy(b->d_inst, h.data(), sizeof(BatchInst) * nInst, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (b->d_inst) (void)hipFree(b->d_inst);
+    if (b->queue) (void)hipFree(b->queue);
+    delete b;
+    return fail(ctx, CPAB_ERR_NOMEM, "batch: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return CPAB_OK;
+}
+
+int cpab_batch_sizes(cpab_batch* b, int64_t* nCells, int64_t* nPts, int64_t* tableDoubles, int* maxNC) {
+  if (!b) return CPAB_ERR_ARG;
+  if (nCells) *nCells = b->nCells;
+  if (nPts) *nPts = b->nPts;
+  if (tableDoubles) *tableDoubles = b->tableDoubles;
+  if (maxNC) *maxNC = b->maxNC;
+  return CPAB_OK;
+}
+
+int cpab_batch_prep(cpab_batch* b, const double* As, int sign, double dt, double* tables) {
+  if (!b) return CPAB_ERR_ARG;
+  cpab_ctx* ctx = b->ctx;
+  if (sign != 1 && sign != -1) return fail(ctx, CPAB_ERR_ARG, "sign must be 1 or -1");
+  if (!As || !tables) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  const int64_t nb = (b->nCells + 255) / 256;
+  if (b->dim == 1)
+    hipLaunchKernelGGL(k_batch_prep<1>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, b->d_inst, b->nInst, As,
+                       b->nCells, sign, dt, tables, 12);
+  else if (b->dim == 2)
+    hipLaunchKernelGGL(k_batch_prep<2>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, b->d_inst, b->nInst, As,
+                       b->nCells, sign, dt, tables, 12);
+  else
+    hipLaunchKernelGGL(k_batch_prep<3>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, b->d_inst, b->nInst, As,
+                       b->nCells, sign, dt, tables, 12);
+  HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+int cpab_batch_calc_T(cpab_batch* b, const double* tables, const double* pos0, double* out, double dt, int nT,
+                      int nS, const cpab_batch_sched* sched, int use_graph, int32_t* nfb) {
+  if (!b) return CPAB_ERR_ARG;
+  cpab_ctx* ctx = b->ctx;
+  cpab_batch_sched q = sched ? *sched : cpab_batch_sched{16384, 1024, 4};
+  if (q.tile < 64 || q.block < 64 || q.block > 1024 || (q.block & 63) || q.chunk < 1)
+    return fail(ctx, CPAB_ERR_ARG, "schedule: tile >= 64, block a multiple of 64 up to 1024, chunk >= 1");
+  if (nT < 0 || nS < 0) return fail(ctx, CPAB_ERR_ARG, "negative step counts");
+  if (b->nPts == 0) return CPAB_OK;
+  if (!tables || !pos0 || !out) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  const size_t pb = (size_t)b->nPts * b->dim * sizeof(double);
+  if (overlap(pos0, pb, out, pb)) return fail(ctx, CPAB_ERR_ALIAS, "pts and out cannot point to the same memory");
+  int r = batch_tiles(b, q.tile);
+  if (r) return r;
+  const double h = dt / (double)(nS > 0 ? nS : 1);
+  hipEvent_t a = nullptr, e = nullptr;
+  if ((r = prof_begin(ctx, &a, &e))) return r;
+  if (!use_graph) {
+    if ((r = batch_dispatch(b, ctx->stream, q, tables, pos0, out, h, nT, nS, nfb))) return r;
+  } else {
+    const void* key[4] = {tables, pos0, out, nfb};
+    const int key_i[5] = {nT, nS, q.tile, q.block, q.chunk};
+    bool same = b->exec && b->key_h == h;
+    for (int i = 0; i < 4 && same; i++) same = b->key[i] == key[i];
+    for (int i = 0; i < 5 && same; i++) same = b->key_i[i] == key_i[i];
+    if (!same) {
+      batch_drop_graph(b);
+      if (!b->cap) HIP_OK(ctx, hipStreamCreate(&b->cap));
+      HIP_OK(ctx, hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+      r = batch_dispatch(b, b->cap, q, tables, pos0, out, h, nT, nS, nfb);
+      hipGraph_t gph = nullptr;
+      hipError_t ee = hipStreamEndCapture(b->cap, &gph);
+      if (r) return r;
+      if (ee != hipSuccess) return fail(ctx, CPAB_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+      b->graph = gph;
+      HIP_OK(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      for (int i = 0; i < 4; i++) b->key[i] = key[i];
+      for (int i = 0; i < 5; i++) b->key_i[i] = key_i[i];
+      b->key_h = h;
+    }
+    HIP_OK(ctx, hipGraphLaunch(b->exec, ctx->stream));
+  }
+  return prof_end(ctx, a, e);
+}
+
+int cpab_batch_destroy(cpab_batch* b) {
+  if (!b) return CPAB_OK;
+  (void)hipStreamSynchronize(b->ctx->stream);
+  batch_drop_graph(b);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
+  if (b->d_tiles) (void)hipFree(b->d_tiles);
+  if (b->d_inst) (void)hipFree(b->d_inst);
+  if (b->queue) (void)hipFree(b->queue);
+  delete b;
+  return CPAB_OK;
+}
+
+// ------------------------------------------------------------- work items
+}  // extern "C"
+
+namespace {
+
+template <int DIM, int TESS, bool POW2>
+int items_launch(cpab_ctx* ctx, int nItems, int maxNC, const cpab_batch_sched& q, double h, int nT, int nS) {
+  constexpr int NE = DIM * (DIM + 1);
+  const size_t lds = (size_t)2 * maxNC * NE * sizeof(double);
+  auto k = k_integrate_items<DIM, TESS, POW2>;
+  allow_lds(k, lds);
+  int per_cu = 0;
+  HIP_OK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, q.block, lds));
+  if (per_cu < 1) return fail(ctx, CPAB_ERR_UNSUPPORTED, "the item kernel does not fit a CU with block %d", q.block);
+  int grid = ctx->n_cu * per_cu;
+  const int64_t chunks = ((int64_t)nItems + q.chunk - 1) / q.chunk;
+  if (grid > chunks) grid = (int)chunks;
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->queue);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(q.block), lds, ctx->stream, ctx->items, nItems, q.chunk, ctx->queue, h, nT,
+                     nS);
+  HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpab_calc_T_items(cpab_ctx* ctx, int dim, int tess, int nItems, const cpab_work_item* items, double dt,
+                      int nTimeSteps, int nStepsODE, const cpab_batch_sched* sched) {
+  if (!ctx) return CPAB_ERR_ARG;
+  if (nItems < 0 || (nItems > 0 && !items)) return fail(ctx, CPAB_ERR_ARG, "bad item list");
+  if (nItems == 0) return CPAB_OK;
+  if (nTimeSteps < 0 || nStepsODE < 0) return fail(ctx, CPAB_ERR_ARG, "negative step counts");
+  cpab_batch_sched q = sched ? *sched : cpab_batch_sched{16384, 1024, 1};
+  if (q.block < 64 || q.block > 1024 || (q.block & 63) || q.chunk < 1)
+    return fail(ctx, CPAB_ERR_ARG, "schedule: block a multiple of 64 up to 1024, chunk >= 1");
+  const int NE = dim * (dim + 1);
+  std::vector<WorkItem>& hv = ctx->h_items;
+  hv.resize(nItems);
+  int maxNC = 0;
+  bool pow2_all = true, pow2_any = false;
+  for (int i = 0; i < nItems; i++) {
+    const cpab_work_item& q2 = items[i];
+    WorkItem& W = hv[i];
+    std::memset(&W, 0, sizeof W);
+    bool pow2;
+    int r = make_geom(ctx, dim, tess, q2.nC0, q2.nC1, q2.nC2, q2.inc_x, q2.inc_y, q2.inc_z, &W.g, &pow2);
+    if (r) return r;
+    int64_t boxes = q2.nC0;
+    if (dim >= 2) boxes *= q2.nC1;
+    if (dim >= 3) boxes *= q2.nC2;
+    const int64_t nC = boxes * (tess == TESS_II ? 1 : (dim == 2 ? 4 : 5));
+    if ((r = check_nC(ctx, dim, tess, (int)nC, q2.nC0, q2.nC1, q2.nC2))) return r;
+    if (!q2.tables || !q2.pos0 || !q2.out || q2.begin < 0 || q2.end < q2.begin)
+      return fail(ctx, CPAB_ERR_ARG, "item %d: null pointer or bad range", i);
+    pow2_all = pow2_all && pow2;
+    pow2_any = pow2_any || pow2;
+    W.nC = (int)nC;
+    W.tables = q2.tables;
+    W.pos0 = q2.pos0;
+    W.out = q2.out;
+    W.begin = q2.begin;
+    W.end = q2.end;
+    if (nC > maxNC) maxNC = (int)nC;
+  }
+  if (pow2_any && !pow2_all)
+    return fail(ctx, CPAB_ERR_UNSUPPORTED, "items mix power-of-two and other increments: launch each kind apart");
+  if ((size_t)2 * maxNC * NE * sizeof(double) > (size_t)ctx->max_lds)
+    return fail(ctx, CPAB_ERR_UNSUPPORTED, "the largest tessellation's tables exceed LDS");
+  if (!ctx->queue) HIP_OK(ctx, hipMalloc(&ctx->queue, 256));
+  if (ctx->items_cap < (size_t)nItems) {
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->items) HIP_OK(ctx, hipFree(ctx->items));
+    ctx->items = nullptr;
+    size_t cap = 1024;
+    while (cap < (size_t)nItems) cap *= 2;
+    HIP_OK(ctx, hipMalloc(&ctx->items, cap * sizeof(WorkItem)));
+    ctx->items_cap = cap;
+  }
+  // pageable source: the copy is staged before the call returns, so the host
+  // vector can be refilled by the next call; stream order protects the device
+  // buffer from a launch still reading it
+  HIP_OK(ctx, hipMemcpyAsync(ctx->items, hv.data(), sizeof(WorkItem) * nItems, hipMemcpyHostToDevice, ctx->stream));
+  const double h = dtdiv(dt, nStepsODE);
+  hipEvent_t a = nullptr, e = nullptr;
+  int r = prof_begin(ctx, &a, &e);
+  if (r) return r;
+#define ITEMS_CASE(D_, T_, P_) \
+  if (dim == D_ && tess == T_ && pow2_all == P_) r = items_launch<D_, T_, P_>(ctx, nItems, maxNC, q, h, nTimeSteps, nStepsODE); else
+  ITEMS_CASE(1, TESS_II, true)
+  ITEMS_CASE(1, TESS_II, false)
+  ITEMS_CASE(2, TESS_I, true)
+  ITEMS_CASE(2, TESS_I, false)
+  ITEMS_CASE(2, TESS_II, true)
+  ITEMS_CASE(2, TESS_II, false)
+  ITEMS_CASE(3, TESS_I, true)
+  ITEMS_CASE(3, TESS_I, false)
+  ITEMS_CASE(3, TESS_II, true)
+  ITEMS_CASE(3, TESS_II, false)
+  r = fail(ctx, CPAB_ERR_UNSUPPORTED, "no item kernel for dim %d tess %d", dim, tess);
+#undef ITEMS_CASE
+  if (r) return r;
+  return prof_end(ctx, a, e);
 }
 
 // ------------------------------------------------------------- MCMC

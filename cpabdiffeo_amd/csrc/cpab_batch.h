@@ -1,0 +1,159 @@
+// Many independent calc_T problems in one persistent launch.
+//
+// Each instance has its own tessellation (cell counts, increments), its own
+// tables (Trels then signed As, staged per instance in one table buffer) and
+// its own point range in one concatenated point buffer.  The host cuts every
+// instance into tiles of at most `tile` points (in instance order) and the
+// kernel's blocks take `chunk` consecutive tiles at a time from a queue (one
+// atomic per chunk; a block leaves when the queue is exhausted, so every
+// block ends).  A block keeps the tables of the instance it is working on in
+// LDS and reloads them only when a tile of another instance comes; the waves
+// of a block split the tile's points and integrate them with the lane-skewed
+// loop of k_integrate.  Per point the arithmetic is exactly calc_T's, so a
+// point gives the same bits batched or alone.
+#pragma once
+#include "cpab_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace cpab {
+
+struct BatchInst {
+  Geom g;
+  int nC;
+  int pad;
+  int64_t tab;   // offset (doubles) of the instance's tables: Trels (nC x NE), then signed As
+  int64_t pts;   // offset (points) of the instance's first point
+  int64_t nPts;
+  int64_t cell;  // offset of the instance's first cell in the concatenated As input
+};
+
+struct BatchTile {
+  int inst;
+  int pad;
+  int64_t begin, end;  // point range, absolute indices into the point buffer
+};
+
+// Signed As and Trels of every cell of every instance, into the table buffer.
+// One lane per cell of the concatenated input As (cells in instance order).
+template <int DIM>
+__global__ void k_batch_prep(const BatchInst* __restrict__ insts, int nInst, const double* __restrict__ As,
+                             int64_t nCells, int sign, double dt, double* __restrict__ tables, int p) {
+  constexpr int NE = DIM * (DIM + 1);
+  const int64_t cg = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cg >= nCells) return;
+  int lo = 0, hi = nInst - 1;  // the instance holding cell cg
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (insts[mid].cell <= cg) lo = mid;
+    else hi = mid - 1;
+  }
+  const BatchInst& I = insts[lo];
+  const int64_t c = cg - I.cell;
+  double sA[NE], T[NE];
+  prep_cell<DIM>(As + cg * NE, sign, dt, sA, T, p);
+  double* Tt = tables + I.tab;
+#pragma unroll
+  for (int k = 0; k < NE; k++) {
+    Tt[c * NE + k] = T[k];
+    Tt[(int64_t)I.nC * NE + c * NE + k] = sA[k];
+  }
+}
+
+__global__ void k_batch_reset(unsigned int* queue) {
+  if (threadIdx.x == 0) *queue = 0u;
+}
+
+template <int DIM, int TESS, bool POW2>
+__global__ void __launch_bounds__(1024)
+    k_integrate_batch(const BatchInst* __restrict__ insts, const BatchTile* __restrict__ tiles, int nTiles, int chunk,
+                      unsigned int* __restrict__ queue, const double* __restrict__ tables,
+                      const double* __restrict__ pos0, double* __restrict__ out, double h, int nT, int nS,
+                      int32_t* __restrict__ nfb_out) {
+  constexpr int NE = DIM * (DIM + 1);
+  extern __shared__ double lds[];
+  __shared__ int s_t0;
+  const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
+  int cur = -1;  // instance whose tables are in LDS (block-uniform)
+  for (;;) {
+    __syncthreads();  // every wave is done with the previous chunk's s_t0
+    if (threadIdx.x == 0) s_t0 = (int)atomicAdd(queue, (unsigned int)chunk);
+    __syncthreads();
+    const int t0 = s_t0;
+    if (t0 >= nTiles) break;
+    const int t1 = t0 + chunk < nTiles ? t0 + chunk : nTiles;
+    for (int t = t0; t < t1; t++) {
+      const BatchTile tl = tiles[t];
+      const int nC = insts[tl.inst].nC;
+      if (tl.inst != cur) {
+        const double* src = tables + insts[tl.inst].tab;
+        __syncthreads();  // the previous tables are no longer read
+        for (int i = threadIdx.x; i < 2 * nC * NE; i += blockDim.x) lds[i] = src[i];
+        __syncthreads();
+        cur = tl.inst;
+      }
+      const Geom g = insts[tl.inst].g;
+      const int64_t n = tl.end - tl.begin;
+      const int64_t per = (n + nw - 1) / nw;
+      const int64_t b = tl.begin + (int64_t)w * per;
+      const int64_t e = b + per < tl.end ? b + per : tl.end;
+      if (b < e)
+        integrate_range<DIM, TESS, POW2, 0>(b, e, pos0, out, lds, lds + nC * NE, h, nT, nS, 0, g, nfb_out);
+    }
+  }
+}
+
+}  // namespace cpab
+
+namespace cpab {
+
+// A tile of one request of a served mix: its own tables, points and
+// geometry, so a launch can take tiles of any set of requests.
+struct WorkItem {
+  Geom g;
+  int nC;
+  int pad;
+  const double* tables;  // Trels (nC x NE) then signed As
+  const double* pos0;
+  double* out;
+  int64_t begin, end;    // point range within the request's arrays
+};
+
+template <int DIM, int TESS, bool POW2>
+__global__ void __launch_bounds__(1024)
+    k_integrate_items(const WorkItem* __restrict__ items, int nItems, int chunk, unsigned int* __restrict__ queue,
+                      double h, int nT, int nS) {
+  constexpr int NE = DIM * (DIM + 1);
+  extern __shared__ double lds[];
+  __shared__ int s_t0;
+  const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
+  const double* cur = nullptr;  // tables in LDS (block-uniform)
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_t0 = (int)atomicAdd(queue, (unsigned int)chunk);
+    __syncthreads();
+    const int t0 = s_t0;
+    if (t0 >= nItems) break;
+    const int t1 = t0 + chunk < nItems ? t0 + chunk : nItems;
+    for (int t = t0; t < t1; t++) {
+      const WorkItem& it = items[t];
+      const int nC = it.nC;
+      const double* tab = it.tables;
+      if (tab != cur) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < 2 * nC * NE; i += blockDim.x) lds[i] = tab[i];
+        __syncthreads();
+        cur = tab;
+      }
+      const Geom g = it.g;
+      const int64_t n = it.end - it.begin;
+      const int64_t per = (n + nw - 1) / nw;
+      const int64_t b = it.begin + (int64_t)w * per;
+      const int64_t e = b + per < it.end ? b + per : it.end;
+      if (b < e)
+        integrate_range<DIM, TESS, POW2, 0>(b, e, it.pos0, it.out, lds, lds + nC * NE, h, nT, nS, 0, g, nullptr);
+    }
+  }
+}
+
+}  // namespace cpab

@@ -117,17 +117,13 @@ __device__ inline void expm_affine3(const double* A, double* T, int p) {
 // Trels = expm(As_dt); in 1D the closed form of :285-292.  One lane per
 // cell; `inst` strides over independent instances (batched callers).
 template <int DIM>
-__global__ void k_prep(const double* __restrict__ As, int nC, int sign, double dt,
-                       double* __restrict__ sA_out, double* __restrict__ T_out, int p,
-                       int64_t inst_stride) {
+__device__ __forceinline__ void prep_cell(const double* __restrict__ A, int sign, double dt, double* sA,
+                                          double* T, int p) {
   constexpr int NE = DIM * (DIM + 1);
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nC) return;
-  const int64_t off = (int64_t)blockIdx.y * inst_stride + (int64_t)c * NE;
-  double sA[NE], sAdt[NE], T[NE];
+  double sAdt[NE];
 #pragma unroll
   for (int k = 0; k < NE; k++) {
-    double v = As[off + k];
+    double v = A[k];
     if (sign == -1) v *= -1;
     sA[k] = v;
     sAdt[k] = v * dt;
@@ -141,6 +137,18 @@ __global__ void k_prep(const double* __restrict__ As, int nC, int sign, double d
   } else {
     expm_affine3(sAdt, T, p);
   }
+}
+
+template <int DIM>
+__global__ void k_prep(const double* __restrict__ As, int nC, int sign, double dt,
+                       double* __restrict__ sA_out, double* __restrict__ T_out, int p,
+                       int64_t inst_stride) {
+  constexpr int NE = DIM * (DIM + 1);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nC) return;
+  const int64_t off = (int64_t)blockIdx.y * inst_stride + (int64_t)c * NE;
+  double sA[NE], T[NE];
+  prep_cell<DIM>(As + off, sign, dt, sA, T, p);
 #pragma unroll
   for (int k = 0; k < NE; k++) {
     if (sA_out) sA_out[off + k] = sA[k];
@@ -267,27 +275,14 @@ __device__ __forceinline__ bool trip(double* p, int& c, int& k, int& nfb, const 
 // Trips run in an inner loop until some lane finishes its point; only then
 // do lanes retire and refill from the wave's range (the bookkeeping costs
 // ballots and branches, so it stays out of the per-trip path).
-template <int DIM, int TESS, bool POW2, bool LDS, int MODE>
-__global__ void __launch_bounds__(1024)
-    k_integrate(const double* __restrict__ pos0, double* __restrict__ out,
-                const double* __restrict__ Trels, const double* __restrict__ As, double h,
-                int nT, int nS, int64_t nPts, Geom g, int nC, int ptsPerWave,
-                int32_t* __restrict__ nfb_out) {
-  constexpr int NE = DIM * (DIM + 1);
-  extern __shared__ double lds[];
-  const double* Ttab = Trels;
-  const double* Atab = As;
-  if (LDS) {
-    stage_tables<NE>(lds, Trels, As, nC);
-    Ttab = lds;
-    Atab = lds + nC * NE;
-  }
-  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int64_t begin = wave * (int64_t)ptsPerWave;
-  if (begin >= nPts) return;
-  const int64_t end = begin + ptsPerWave < nPts ? begin + ptsPerWave : nPts;
+// One wave integrates the points [begin, end) of pos0 (indices into pos0,
+// out and nfb_out).  Wave-uniform arguments; every lane of the wave calls it.
+template <int DIM, int TESS, bool POW2, int MODE>
+__device__ __forceinline__ void integrate_range(int64_t begin, int64_t end, const double* __restrict__ pos0,
+                                                double* __restrict__ out, const double* Ttab, const double* Atab,
+                                                double h, int nT, int nS, int64_t nPts, const Geom& g,
+                                                int32_t* __restrict__ nfb_out) {
   int64_t next = begin;  // wave-uniform cursor into [begin, end)
-
   double p[DIM];
   int c = 0, t = 0, k = 0, nfb = 0;
   int64_t my = 0;
@@ -323,6 +318,28 @@ __global__ void __launch_bounds__(1024)
       }
     } while (ballot(t >= nT) == 0);
   }
+}
+
+template <int DIM, int TESS, bool POW2, bool LDS, int MODE>
+__global__ void __launch_bounds__(1024)
+    k_integrate(const double* __restrict__ pos0, double* __restrict__ out,
+                const double* __restrict__ Trels, const double* __restrict__ As, double h,
+                int nT, int nS, int64_t nPts, Geom g, int nC, int ptsPerWave,
+                int32_t* __restrict__ nfb_out) {
+  constexpr int NE = DIM * (DIM + 1);
+  extern __shared__ double lds[];
+  const double* Ttab = Trels;
+  const double* Atab = As;
+  if (LDS) {
+    stage_tables<NE>(lds, Trels, As, nC);
+    Ttab = lds;
+    Atab = lds + nC * NE;
+  }
+  const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t begin = wave * (int64_t)ptsPerWave;
+  if (begin >= nPts) return;
+  const int64_t end = begin + ptsPerWave < nPts ? begin + ptsPerWave : nPts;
+  integrate_range<DIM, TESS, POW2, MODE>(begin, end, pos0, out, Ttab, Atab, h, nT, nS, nPts, g, nfb_out);
 }
 
 // ---------------------------------------------------------- sensitivity

@@ -1,0 +1,289 @@
+"""A device-resident request loop with continuous batching for CPA
+transforms (no reference counterpart: the reference transforms one request
+per Calcs.calc_T call, cpab/gpu/Calcs.py:454-561).
+
+Requests (1D point transforms, 2D grid warps, 3D volume warps) arrive over
+time.  A request is admitted when the in-flight device memory (points,
+outputs, tables, volumes) stays within the HBM budget; admission uploads
+nothing but its theta: its points are generated on the device, its Avees =
+B theta and its tables (signed As, expm Trels) are computed on the device
+and stay there.  Its points are cut into tiles.  The loop keeps up to
+`inflight` iterations on the GPU; an iteration takes pending tiles of the
+admitted requests in arrival order, up to `max_points` points per dimension
+class, and runs each class as one persistent launch (cpab_calc_T_items), so
+requests that arrive while others run join the next iteration and their work
+overlaps on the device.  A request completes when its last tile has run (3D:
+then its trilinear remap).  Per point the arithmetic is calc_T's, so a
+result is the same alone and in the mix.
+"""
+import collections
+import ctypes
+import time
+
+import numpy as np
+
+from ._lib import Context
+from .gpuarray import require_gpu
+from .space import CpaSpace
+
+
+class WorkItem(ctypes.Structure):
+    """cpab_work_item of include/cpab_hip.h."""
+    _fields_ = [("tables", ctypes.c_void_p), ("pos0", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("begin", ctypes.c_int64), ("end", ctypes.c_int64),
+                ("nC0", ctypes.c_int), ("nC1", ctypes.c_int), ("nC2", ctypes.c_int),
+                ("inc_x", ctypes.c_double), ("inc_y", ctypes.c_double), ("inc_z", ctypes.c_double)]
+
+
+class Sched(ctypes.Structure):
+    _fields_ = [("tile", ctypes.c_int), ("block", ctypes.c_int), ("chunk", ctypes.c_int)]
+
+
+VOL = 64  # 3D volumes are VOL^3 float32
+
+
+class Kind:
+    """One request type: its space, prior factor, sign and launch class."""
+
+    def __init__(self, name, space, length_scale, sign, cls):
+        self.name, self.space, self.sign, self.cls = name, space, sign, cls
+        cov = space.se_prior_cov(length_scale, 1.0)
+        w, U = np.linalg.eigh(cov)
+        self.factor = U * np.sqrt(np.clip(w, 0.0, None))
+        self.d = space.d
+        self.dim = space.dim_domain
+        self.ne = self.dim * (self.dim + 1)
+        T = space.tessellation
+        nCs = list(T.nCs) + [0] * (3 - self.dim)
+        incs = [float(x) for x in T.incs] + [1.0] * (3 - self.dim)
+        self.geo = (int(nCs[0]), int(nCs[1]), int(nCs[2]), incs[0], incs[1], incs[2])
+
+
+def default_kinds():
+    """The served mix's tessellations: 1D 50 intervals on [0, 1] (increment
+    from the centre differences, 0.02), 2D 4x4 and 8x8 rectangles x 4
+    triangles on [0, 1]^2, 3D 4x4x4 cubes x 5 tetrahedra on [0, 64]^3; zero
+    normal velocity on the boundary."""
+    return {
+        "1d": Kind("1d", CpaSpace([0], [1], [50], zero_v_across_bdry=[True], tess="II", valid_outside=False),
+                   0.1, 1, (1, "II")),
+        "2d4": Kind("2d4", CpaSpace([0, 0], [1, 1], [4, 4], zero_v_across_bdry=[True] * 2, tess="I",
+                                    valid_outside=False), 0.1, 1, (2, "I")),
+        "2d8": Kind("2d8", CpaSpace([0, 0], [1, 1], [8, 8], zero_v_across_bdry=[True] * 2, tess="I",
+                                    valid_outside=False), 0.1, 1, (2, "I")),
+        "3d": Kind("3d", CpaSpace([0] * 3, [VOL] * 3, [4, 4, 4], zero_v_across_bdry=[True] * 3, tess="I",
+                                  valid_outside=False), 0.1, -1, (3, "I")),
+    }
+
+
+class Request:
+    def __init__(self, rid, kind, arrival, n, side, seed, theta):
+        self.id, self.kind, self.arrival, self.n, self.side, self.seed = rid, kind, arrival, n, side, seed
+        self.theta = theta
+        self.t_admit = self.t_done = None
+        self.result = None
+
+
+def make_mix(n_req, kinds, rate=2000.0, seed=0, n1d=(1e4, 1e6), sides=(64, 128, 256, 512, 1024)):
+    """Poisson arrivals at `rate`; 40 % 1D (N ~ logU n1d), 40 % 2D grids
+    (side from `sides`, tessellation 4x4 or 8x8), 20 % 3D volumes; theta from
+    the matching prior."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    reqs = []
+    for i in range(n_req):
+        u = rng.random()
+        if u < 0.4:
+            k, n, side = "1d", int(round(np.exp(rng.uniform(np.log(n1d[0]), np.log(n1d[1]))))), 0
+        elif u < 0.8:
+            k = "2d4" if rng.random() < 0.5 else "2d8"
+            side = int(rng.choice(sides))
+            n = side * side
+        else:
+            k, n, side = "3d", VOL ** 3, 0
+        theta = kinds[k].factor @ rng.standard_normal(kinds[k].d)
+        reqs.append(Request(i, k, float(t[i]), n, side, int(rng.integers(1 << 31)), theta))
+    return reqs
+
+
+class Server:
+    def __init__(self, kinds=None, budget_bytes=64 << 30, max_points=1 << 22, tile=16384, block=1024,
+                 inflight=2, params=None, n_volumes=4, seed=0, device=0):
+        torch = require_gpu()
+        self.torch = torch
+        self.kinds = kinds or default_kinds()
+        self.budget = int(budget_bytes)
+        self.max_points, self.tile, self.inflight = int(max_points), int(tile), int(inflight)
+        self.params = params or dict(dt=0.01, nTimeSteps=100, nStepsODEsolver=10)
+        self.sched = Sched(tile, block, 1)
+        self.ctx = Context.get(device)
+        self.B = {k: torch.from_numpy(np.ascontiguousarray(v.space.B)).cuda() for k, v in self.kinds.items()}
+        from . import configs
+        self.volumes = [torch.from_numpy(configs.blob_image(VOL, 40, seed=seed + i, dims=3).astype(np.float32))
+                        .cuda() for i in range(n_volumes)]
+        zz, yy, xx = np.mgrid[0:VOL, 0:VOL, 0:VOL]
+        self.voxels = torch.from_numpy(np.stack([xx.ravel(), yy.ravel(), zz.ravel()], 1).astype(np.float64)).cuda()
+
+    # ------------------------------------------------------------ request state
+    def bytes_of(self, r):
+        k = self.kinds[r.kind]
+        b = 2 * r.n * k.dim * 8 + 2 * k.space.nC * k.ne * 8
+        if r.kind == "3d":
+            b = r.n * 3 * 8 + 2 * k.space.nC * k.ne * 8 + r.n * 4  # voxel grid shared; out + warped
+        return b
+
+    def _points(self, r):
+        torch = self.torch
+        if r.kind == "1d":
+            g = torch.Generator(device="cuda")
+            g.manual_seed(r.seed)
+            return torch.rand((r.n, 1), dtype=torch.float64, device="cuda", generator=g)
+        if r.kind == "3d":
+            return self.voxels
+        s = r.side
+        ar = torch.arange(s, dtype=torch.float64, device="cuda") / s
+        yy, xx = torch.meshgrid(ar, ar, indexing="ij")
+        return torch.stack([xx.reshape(-1), yy.reshape(-1)], 1).contiguous()
+
+    def admit(self, r):
+        """Device state of a request: points, output, tables from theta."""
+        torch = self.torch
+        k = self.kinds[r.kind]
+        r.pts = self._points(r)
+        r.out = torch.empty_like(r.pts)
+        th = torch.from_numpy(r.theta).cuda()
+        As = (self.B[r.kind] @ th).reshape(k.space.nC, k.ne).contiguous()
+        nce = k.space.nC * k.ne
+        r.tables = torch.empty(2 * nce, dtype=torch.float64, device="cuda")
+        self.ctx.bind_stream(torch.cuda.current_stream().cuda_stream)
+        self.ctx.call("cpab_prep_trels", k.dim, k.space.nC, As.data_ptr(), int(k.sign), float(self.params["dt"]),
+                      r.tables.data_ptr() + nce * 8, r.tables.data_ptr())
+        r.tiles = collections.deque((b, min(b + self.tile, r.n)) for b in range(0, r.n, self.tile))
+        r.left = len(r.tiles)
+        r.warped = None
+        if r.kind == "3d":
+            r.warped = torch.empty((VOL, VOL, VOL), dtype=torch.float32, device="cuda")
+
+    def finish(self, r):
+        """After the last tile: the 3D warp's remap (trilinear, calc_T_inv points)."""
+        if r.kind == "3d":
+            vol = self.volumes[r.seed % len(self.volumes)]
+            self.ctx.call("cpab_remap3d", r.out.data_ptr(), vol.data_ptr(), r.warped.data_ptr(), VOL, VOL, VOL)
+
+    # ------------------------------------------------------------ iterations
+    def launch(self, active):
+        """One iteration: per launch class, pending tiles in arrival order up
+        to max_points, one persistent launch.  Returns [(request, tiles)]."""
+        torch = self.torch
+        taken = []
+        by_cls = collections.OrderedDict()
+        for r in active:
+            if r.tiles:
+                by_cls.setdefault(self.kinds[r.kind].cls, []).append(r)
+        self.ctx.bind_stream(torch.cuda.current_stream().cuda_stream)
+        p = self.params
+        for (dim, tess), reqs in by_cls.items():
+            items, budget = [], self.max_points
+            for r in reqs:
+                k = self.kinds[r.kind]
+                cnt = 0
+                while r.tiles and budget > 0:
+                    b, e = r.tiles.popleft()
+                    items.append(WorkItem(r.tables.data_ptr(), r.pts.data_ptr(), r.out.data_ptr(), b, e, *k.geo))
+                    budget -= e - b
+                    cnt += 1
+                if cnt:
+                    taken.append((r, cnt))
+                if budget <= 0:
+                    break
+            arr = (WorkItem * len(items))(*items)
+            self.ctx.call("cpab_calc_T_items", dim, 1 if tess == "I" else 2, len(items), arr, float(p["dt"]),
+                          int(p["nTimeSteps"]), int(p["nStepsODEsolver"]), ctypes.byref(self.sched))
+        # requests whose last tile is in this iteration: their remap follows
+        # in stream order, before the iteration's completion event
+        for r, _ in taken:
+            if not r.tiles:
+                self.finish(r)
+        return taken
+
+    def run(self, requests, keep=False, realtime=True):
+        """Serve `requests` (sorted by arrival).  Returns a stats dict; with
+        keep, every request keeps its result tensors in .result."""
+        torch = self.torch
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = collections.deque()
+        active = []
+        flights = collections.deque()
+        used = peak = 0
+        iters = shared = 0
+        done = []
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        while pending or waiting or active or flights:
+            t = now()
+            while pending and (pending[0].arrival <= t or not realtime):
+                waiting.append(pending.popleft())
+            while waiting and used + self.bytes_of(waiting[0]) <= self.budget:
+                r = waiting.popleft()
+                self.admit(r)
+                r.t_admit = now()
+                used += self.bytes_of(r)
+                peak = max(peak, used)
+                active.append(r)
+            if len(flights) < self.inflight and any(r.tiles for r in active):
+                taken = self.launch(active)
+                iters += 1
+                shared += len(taken) > 1
+                ev = torch.cuda.Event()
+                ev.record()
+                flights.append((ev, taken))
+            progressed = False
+            while flights and flights[0][0].query():
+                ev, taken = flights.popleft()
+                progressed = True
+                for r, cnt in taken:
+                    r.left -= cnt
+                    if r.left == 0:
+                        r.t_done = now()
+                        used -= self.bytes_of(r)
+                        active.remove(r)
+                        if keep:
+                            r.result = r.warped if r.kind == "3d" else r.out
+                        else:
+                            r.pts = r.out = r.tables = r.warped = None
+                        done.append(r)
+            if not progressed and not (len(flights) < self.inflight and any(r.tiles for r in active)):
+                if flights:
+                    time.sleep(2e-5)
+                elif pending and realtime:
+                    time.sleep(max(0.0, min(pending[0].arrival - now(), 1e-3)))
+        makespan = now()
+        lat = np.array([r.t_done - r.arrival for r in done])
+        pts = sum(r.n for r in done)
+        per_kind = {}
+        for k in sorted(set(r.kind for r in done)):
+            lk = np.array([r.t_done - r.arrival for r in done if r.kind == k])
+            per_kind[k] = {"n": int(len(lk)), "p50_ms": float(np.percentile(lk, 50) * 1e3),
+                           "p99_ms": float(np.percentile(lk, 99) * 1e3)}
+        return {"requests": len(done), "makespan_s": makespan, "requests_per_sec": len(done) / makespan,
+                "points_per_sec": pts / makespan, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_ms": float(np.percentile(lat, 99) * 1e3), "peak_bytes": int(peak), "iterations": iters,
+                "iterations_with_several_requests": shared, "per_kind": per_kind}
+
+    def alone(self, r):
+        """The same request through the single-call path: update_pat +
+        calc_T (+ remap3d), for comparison with its result in the mix."""
+        torch = self.torch
+        k = self.kinds[r.kind]
+        sp = k.space
+        pts = self._points(r)
+        out = torch.empty_like(pts)
+        sp.update_pat((self.B[r.kind] @ torch.from_numpy(r.theta).cuda()).cpu().numpy())
+        sp._calc_T(pts=pts, mysign=k.sign, out=out, **self.params)
+        if r.kind != "3d":
+            return out
+        warped = torch.empty((VOL, VOL, VOL), dtype=torch.float32, device="cuda")
+        vol = self.volumes[r.seed % len(self.volumes)]
+        self.ctx.bind_stream(torch.cuda.current_stream().cuda_stream)
+        self.ctx.call("cpab_remap3d", out.data_ptr(), vol.data_ptr(), warped.data_ptr(), VOL, VOL, VOL)
+        return warped

@@ -155,6 +155,69 @@ int cpab_ssd_ll(cpab_ctx *ctx, const float *x, const float *y, int64_t n, double
                 double *ll);
 
 /* ------------------------------------------------------------------------
+ * Batched calc_T over many independent instances (no reference counterpart:
+ * the reference makes one Calcs.calc_T call per instance, cpab/gpu/Calcs.py:454-561;
+ * this runs a list of such calls as one persistent launch).  Every instance
+ * has its own tessellation (same dim and tess across the batch; increments
+ * all powers of two or none), tables and points; the points of all
+ * instances are concatenated in instance order, as are their unsigned As
+ * (cells in instance order) for cpab_batch_prep.  Per point the result is
+ * bit-identical to cpab_calc_T on that instance alone. */
+typedef struct {
+  int nC0, nC1, nC2;
+  double inc_x, inc_y, inc_z;
+  int64_t nPts;
+} cpab_batch_inst;
+
+/* Schedule of the persistent launch: points per tile, threads per block
+ * (a multiple of 64, at most 1024), tiles a block takes per queue grab. */
+typedef struct {
+  int tile;
+  int block;
+  int chunk;
+} cpab_batch_sched;
+
+typedef struct cpab_batch cpab_batch;
+
+int cpab_batch_create(cpab_ctx *ctx, int dim, int tess, int nInst, const cpab_batch_inst *insts,
+                      cpab_batch **out);
+/* Total cells, total points, doubles of the table buffer, largest nC. */
+int cpab_batch_sizes(cpab_batch *b, int64_t *nCells, int64_t *nPts, int64_t *tableDoubles,
+                     int *maxNC);
+/* prepare_signedSqAs_and_Tlocals_for_gpu (cpab/cpaNd/_CpaCalcs.py:227-313) for
+ * every instance at once: As (nCells, dim(dim+1)) unsigned tables in
+ * instance order -> tables (per instance: Trels then sign*As). */
+int cpab_batch_prep(cpab_batch *b, const double *As, int sign, double dt, double *tables);
+/* calc_T of every instance (cpab/gpu/calc_2Dto2D.cu:348-408 per point).
+ * sched NULL: {16384, 1024, 4}.  use_graph != 0: the launch is captured once
+ * per argument set and replayed.  nfb (optional, int32 per point): rejected
+ * Trel proposals. */
+int cpab_batch_calc_T(cpab_batch *b, const double *tables, const double *pos0, double *out,
+                      double dt, int nTimeSteps, int nStepsODE, const cpab_batch_sched *sched,
+                      int use_graph, int32_t *nfb);
+int cpab_batch_destroy(cpab_batch *b);
+
+/* One tile of a served request for cpab_calc_T_items: the request's own
+ * tables (Trels then signed As, as cpab_prep_trels writes them when
+ * signedAs_out = Trels_out + nC*dim*(dim+1)), points and output, the point
+ * range [begin, end) of this tile, and the request's tessellation. */
+typedef struct {
+  const double *tables;
+  const double *pos0;
+  double *out;
+  int64_t begin, end;
+  int nC0, nC1, nC2;
+  double inc_x, inc_y, inc_z;
+} cpab_work_item;
+
+/* calc_T over a list of tiles of any requests of one dim and tess (no
+ * reference counterpart: the serving loop's continuous batching), one
+ * persistent launch; per point bit-identical to cpab_calc_T.  Increments of
+ * all items powers of two or none.  sched NULL: {16384 (unused), 1024, 1}. */
+int cpab_calc_T_items(cpab_ctx *ctx, int dim, int tess, int nItems, const cpab_work_item *items,
+                      double dt, int nTimeSteps, int nStepsODE, const cpab_batch_sched *sched);
+
+/* ------------------------------------------------------------------------
  * Device-resident Metropolis sampling of 2D image registration, for many
  * independent chains at once: Metropolis.run / .step
  * (cpab/cpaNd/inference/Metropolis.py:41-151) with the random-walk proposal

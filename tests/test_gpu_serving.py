@@ -1,0 +1,48 @@
+"""The serving loop: every result equals the same request served alone
+(single calc_T call + remap), the HBM budget holds, and iterations mix
+requests (continuous batching)."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def kinds(gpu):
+    from cpabdiffeo_amd import serving
+    return serving.default_kinds()
+
+
+def test_results_equal_alone_and_in_the_mix(kinds):
+    import torch
+    from cpabdiffeo_amd import serving
+    reqs = serving.make_mix(40, kinds, rate=4000.0, seed=1, n1d=(1e3, 3e4), sides=(32, 64, 128))
+    srv = serving.Server(kinds, max_points=1 << 16, tile=4096, params=dict(dt=0.02, nTimeSteps=50, nStepsODEsolver=10))
+    st = srv.run(reqs, keep=True)
+    say("served", st)
+    assert st["requests"] == len(reqs)
+    assert st["iterations_with_several_requests"] > 0
+    for r in reqs:
+        alone = srv.alone(r)
+        torch.cuda.synchronize()
+        assert torch.equal(alone, r.result), (r.id, r.kind)
+
+
+def test_budget_is_respected(kinds):
+    from cpabdiffeo_amd import serving
+    reqs = serving.make_mix(30, kinds, rate=1e6, seed=2, n1d=(1e3, 1e4), sides=(32, 64))
+    budget = 12 << 20
+    srv = serving.Server(kinds, budget_bytes=budget, max_points=1 << 15, tile=2048,
+                         params=dict(dt=0.02, nTimeSteps=20, nStepsODEsolver=10))
+    assert max(srv.bytes_of(r) for r in reqs) <= budget
+    st = srv.run(reqs, realtime=False)
+    say("budget run", st)
+    assert st["requests"] == len(reqs)
+    assert st["peak_bytes"] <= budget
