@@ -959,6 +959,18 @@ int items_launch(cpab_ctx* ctx, int nItems, int maxNC, const cpab_batch_sched& q
 
 extern "C" {
 
+int cpab_theta2avees(cpab_ctx* ctx, int D, int d, int m, const double* B, const double* theta, double* Av) {
+  if (!ctx) return CPAB_ERR_ARG;
+  if (D < 1 || d < 1 || m < 0) return fail(ctx, CPAB_ERR_ARG, "bad sizes");
+  if (m == 0) return CPAB_OK;
+  if (!B || !theta || !Av) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  const int64_t n = (int64_t)m * D;
+  hipLaunchKernelGGL(k_theta2avees, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, B, theta, Av, D, d,
+                     m);
+  HIP_OK(ctx, hipGetLastError());
+  return CPAB_OK;
+}
+
 int cpab_calc_T_items(cpab_ctx* ctx, int dim, int tess, int nItems, const cpab_work_item* items, double dt,
                       int nTimeSteps, int nStepsODE, const cpab_batch_sched* sched) {
   if (!ctx) return CPAB_ERR_ARG;
@@ -985,12 +997,13 @@ int cpab_calc_T_items(cpab_ctx* ctx, int dim, int tess, int nItems, const cpab_w
     if (dim >= 3) boxes *= q2.nC2;
     const int64_t nC = boxes * (tess == TESS_II ? 1 : (dim == 2 ? 4 : 5));
     if ((r = check_nC(ctx, dim, tess, (int)nC, q2.nC0, q2.nC1, q2.nC2))) return r;
-    if (!q2.tables || !q2.pos0 || !q2.out || q2.begin < 0 || q2.end < q2.begin)
+    if (!q2.trels || !q2.As || !q2.pos0 || !q2.out || q2.begin < 0 || q2.end < q2.begin)
       return fail(ctx, CPAB_ERR_ARG, "item %d: null pointer or bad range", i);
     pow2_all = pow2_all && pow2;
     pow2_any = pow2_any || pow2;
     W.nC = (int)nC;
-    W.tables = q2.tables;
+    W.trels = q2.trels;
+    W.as = q2.As;
     W.pos0 = q2.pos0;
     W.out = q2.out;
     W.begin = q2.begin;

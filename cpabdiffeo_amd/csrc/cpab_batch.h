@@ -113,7 +113,8 @@ struct WorkItem {
   Geom g;
   int nC;
   int pad;
-  const double* tables;  // Trels (nC x NE) then signed As
+  const double* trels;   // (nC, NE)
+  const double* as;      // (nC, NE) signed As
   const double* pos0;
   double* out;
   int64_t begin, end;    // point range within the request's arrays
@@ -127,7 +128,7 @@ __global__ void __launch_bounds__(1024)
   extern __shared__ double lds[];
   __shared__ int s_t0;
   const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
-  const double* cur = nullptr;  // tables in LDS (block-uniform)
+  const double* cur = nullptr;  // Trels in LDS (block-uniform)
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_t0 = (int)atomicAdd(queue, (unsigned int)chunk);
@@ -138,12 +139,16 @@ __global__ void __launch_bounds__(1024)
     for (int t = t0; t < t1; t++) {
       const WorkItem& it = items[t];
       const int nC = it.nC;
-      const double* tab = it.tables;
-      if (tab != cur) {
+      const double* tr = it.trels;
+      if (tr != cur) {
+        const double* sa = it.as;
         __syncthreads();
-        for (int i = threadIdx.x; i < 2 * nC * NE; i += blockDim.x) lds[i] = tab[i];
+        for (int i = threadIdx.x; i < nC * NE; i += blockDim.x) {
+          lds[i] = tr[i];
+          lds[nC * NE + i] = sa[i];
+        }
         __syncthreads();
-        cur = tab;
+        cur = tr;
       }
       const Geom g = it.g;
       const int64_t n = it.end - it.begin;

@@ -127,19 +127,16 @@ __device__ __forceinline__ int cell2(const double* p, const Geom& g) {
   const double px = p[0], py = p[1];
   const bool left = px <= 0.0, right = px >= g.wall_x, up = py <= 0.0, down = py >= g.wall_y;
   if (left | right | up | down) {  // rare: outside the domain
+    // the reference's rules, as selects on the four sides' tests
     const double nC0 = (double)g.nC0, nC1 = (double)g.nC1;
-    if (left) {
-      if (up && divinc<POW2>(py, g.inc_y, g.rinc_y) < divinc<POW2>(px, g.inc_x, g.rinc_x)) return c;
-      if (down && divinc<POW2>(py, g.inc_y, g.rinc_y) - nC1 > divinc<POW2>(-px, g.inc_x, g.rinc_x)) return c + 2;
-      return c + 3;
-    }
-    if (right) {
-      if (up && divinc<POW2>(-py, g.inc_y, g.rinc_y) > divinc<POW2>(px, g.inc_x, g.rinc_x) - nC0) return c;
-      if (down && divinc<POW2>(py, g.inc_y, g.rinc_y) - nC1 > divinc<POW2>(px, g.inc_x, g.rinc_x) - nC0) return c + 2;
-      return c + 1;
-    }
-    if (up) return c;
-    return c + 2;
+    const double sx = divinc<POW2>(px, g.inc_x, g.rinc_x), sy = divinc<POW2>(py, g.inc_y, g.rinc_y);
+    const double nx = divinc<POW2>(-px, g.inc_x, g.rinc_x), ny = divinc<POW2>(-py, g.inc_y, g.rinc_y);
+    const bool upL = up & (sy < sx), dnL = down & (sy - nC1 > nx);
+    const bool upR = up & (ny > sx - nC0), dnR = down & (sy - nC1 > sx - nC0);
+    const int tl = upL ? 0 : (dnL ? 2 : 3);
+    const int tr = upR ? 0 : (dnR ? 2 : 1);
+    const int tm = up ? 0 : 2;
+    return c + (left ? tl : (right ? tr : tm));
   }
   const double omx = 1.0 - x;
   if (x < y) return c + ((omx < y) ? 2 : 3);

@@ -156,6 +156,22 @@ __global__ void k_prep(const double* __restrict__ As, int nC, int sign, double d
   }
 }
 
+// Avees = B theta for m thetas (CpaSpace.theta2Avees, cpab/cpaNd/_CpaSpace.py:366-376):
+// one lane per (theta, row), the row's dot product summed in index order, so
+// a theta gives the same Avees whatever else is in the batch.
+__global__ void k_theta2avees(const double* __restrict__ B, const double* __restrict__ theta,
+                              double* __restrict__ Av, int D, int d, int m) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)m * D) return;
+  const int64_t i = idx / D;
+  const int r = (int)(idx - i * D);
+  const double* row = B + (int64_t)r * d;
+  const double* th = theta + i * d;
+  double acc = 0.0;
+  for (int j = 0; j < d; j++) acc += row[j] * th[j];
+  Av[idx] = acc;
+}
+
 // The reference's gpu_expm(As, Ts, p) on an already scaled table.
 template <int DIM>
 __global__ void k_expm(const double* __restrict__ As, double* __restrict__ Ts, int nC, int p) {
@@ -227,18 +243,17 @@ __device__ __forceinline__ void stage_tables(double* lds, const double* __restri
 // lookups overlap, are both slower than this.)
 // Returns true when the step is complete (the caller advances t).
 template <int DIM, int TESS, bool POW2>
-__device__ __forceinline__ bool trip(double* p, int& c, int& k, int& nfb, const double* Ttab,
+__device__ __forceinline__ bool trip(double* p, int& c, int& k, const double*& row, int& nfb, const double* Ttab,
                                      const double* Atab, double h, int nS, const Geom& g) {
   constexpr int NE = DIM * (DIM + 1);
   const bool fast = (k == 0);
-  const double* M = (fast ? Ttab : Atab) + __umul24((unsigned)c, (unsigned)NE);
   double y[DIM];
-  affine<DIM>(y, M, p);
+  affine<DIM>(y, row, p);
   if (!fast) {  // RK2 midpoint substep (solveODE, calc_2Dto2D.cu:182-194)
     double m[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; d++) m[d] = p[d] + h * y[d] / 2.;
-    affine<DIM>(y, M, m);
+    affine<DIM>(y, row, m);
 #pragma unroll
     for (int d = 0; d < DIM; d++) y[d] = p[d] + y[d] * h;
   }
@@ -246,8 +261,10 @@ __device__ __forceinline__ bool trip(double* p, int& c, int& k, int& nfb, const 
   if (!fast) {
 #pragma unroll
     for (int d = 0; d < DIM; d++) p[d] = y[d];
+    const bool done = --k == 0;
+    if (done | (cy != c)) row = (done ? Ttab : Atab) + mul24(cy, NE);
     c = cy;
-    return --k == 0;
+    return done;
   }
   if (cy == c) {
 #pragma unroll
@@ -257,6 +274,7 @@ __device__ __forceinline__ bool trip(double* p, int& c, int& k, int& nfb, const 
   nfb++;
   if (nS > 0) {
     k = nS;
+    row = Atab + mul24(c, NE);
     return false;
   }
   return true;  // zero substeps: the step leaves p where it was
@@ -282,9 +300,11 @@ __device__ __forceinline__ void integrate_range(int64_t begin, int64_t end, cons
                                                 double* __restrict__ out, const double* Ttab, const double* Atab,
                                                 double h, int nT, int nS, int64_t nPts, const Geom& g,
                                                 int32_t* __restrict__ nfb_out) {
+  constexpr int NE = DIM * (DIM + 1);
   int64_t next = begin;  // wave-uniform cursor into [begin, end)
   double p[DIM];
   int c = 0, t = 0, k = 0, nfb = 0;
+  const double* row = Ttab;  // the table row the next trip reads
   int64_t my = 0;
   bool active = false;
   for (;;) {
@@ -302,6 +322,7 @@ __device__ __forceinline__ void integrate_range(int64_t begin, int64_t end, cons
           active = true;
           load_pt<DIM>(pos0, my, p);
           c = cell_of<DIM, TESS, POW2>(p, g);
+          row = Ttab + mul24(c, NE);
           t = 0;
           k = 0;
           nfb = 0;
@@ -311,8 +332,9 @@ __device__ __forceinline__ void integrate_range(int64_t begin, int64_t end, cons
     }
     if (ballot(active) == 0) break;
     if (!active || t >= nT) continue;
+#pragma unroll 2
     do {  // trips until some lane of the region finishes its point
-      if (trip<DIM, TESS, POW2>(p, c, k, nfb, Ttab, Atab, h, nS, g)) {
+      if (trip<DIM, TESS, POW2>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) {
         if (MODE == 1) store_pt<DIM>(out, (int64_t)t * nPts + my, p);
         t++;
       }

@@ -330,6 +330,7 @@ __global__ void __launch_bounds__(256) k_mcmc_step(McmcDesc D, int init) {
       int64_t next = begin;
       double p[2];
       int c = 0, t = 0, k = 0;
+      const double* row = Ttab;
       int64_t my = 0;
       bool active = false;
       for (;;) {
@@ -349,6 +350,7 @@ __global__ void __launch_bounds__(256) k_mcmc_step(McmcDesc D, int init) {
               p[0] = (double)(my % D.W);
               p[1] = (double)(my / D.W);
               c = cell2<TESS, POW2>(p, g);
+              row = Ttab + mul24(c, 6);
               t = 0;
               k = 0;
             }
@@ -358,7 +360,7 @@ __global__ void __launch_bounds__(256) k_mcmc_step(McmcDesc D, int init) {
         if (ballot(active) == 0) break;
         if (!active || t >= nT) continue;
         do {
-          if (trip<2, TESS, POW2>(p, c, k, nfb, Ttab, Atab, h, nS, g)) t++;
+          if (trip<2, TESS, POW2>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) t++;
         } while (ballot(t >= nT) == 0);
       }
     }

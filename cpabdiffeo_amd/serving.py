@@ -27,12 +27,12 @@ from .gpuarray import require_gpu
 from .space import CpaSpace
 
 
-class WorkItem(ctypes.Structure):
-    """cpab_work_item of include/cpab_hip.h."""
-    _fields_ = [("tables", ctypes.c_void_p), ("pos0", ctypes.c_void_p), ("out", ctypes.c_void_p),
-                ("begin", ctypes.c_int64), ("end", ctypes.c_int64),
-                ("nC0", ctypes.c_int), ("nC1", ctypes.c_int), ("nC2", ctypes.c_int),
-                ("inc_x", ctypes.c_double), ("inc_y", ctypes.c_double), ("inc_z", ctypes.c_double)]
+# cpab_work_item of include/cpab_hip.h, as a numpy record so an iteration's
+# item list is built with array operations
+ITEM = np.dtype({"names": ["trels", "As", "pos0", "out", "begin", "end", "nC0", "nC1", "nC2",
+                           "inc_x", "inc_y", "inc_z"],
+                 "formats": [np.uint64] * 4 + [np.int64] * 2 + [np.int32] * 3 + [np.float64] * 3,
+                 "offsets": [0, 8, 16, 24, 32, 40, 48, 52, 56, 64, 72, 80], "itemsize": 88})
 
 
 class Sched(ctypes.Structure):
@@ -118,6 +118,7 @@ class Server:
         self.sched = Sched(tile, block, 1)
         self.ctx = Context.get(device)
         self.B = {k: torch.from_numpy(np.ascontiguousarray(v.space.B)).cuda() for k, v in self.kinds.items()}
+        self._grids = {}
         from . import configs
         self.volumes = [torch.from_numpy(configs.blob_image(VOL, 40, seed=seed + i, dims=3).astype(np.float32))
                         .cuda() for i in range(n_volumes)]
@@ -126,11 +127,24 @@ class Server:
 
     # ------------------------------------------------------------ request state
     def bytes_of(self, r):
+        """Device bytes the request holds while admitted."""
         k = self.kinds[r.kind]
-        b = 2 * r.n * k.dim * 8 + 2 * k.space.nC * k.ne * 8
+        tables = 2 * k.space.nC * k.ne * 8
         if r.kind == "3d":
-            b = r.n * 3 * 8 + 2 * k.space.nC * k.ne * 8 + r.n * 4  # voxel grid shared; out + warped
-        return b
+            return tables + r.n * 3 * 8 + r.n * 4  # output points + warped volume (voxel grid shared)
+        if r.kind == "1d":
+            return tables + 2 * r.n * 8
+        return tables + r.n * 2 * 8  # output (the side's grid is shared)
+
+    def _grid(self, side):
+        g = self._grids.get(side)
+        if g is None:
+            torch = self.torch
+            ar = torch.arange(side, dtype=torch.float64, device="cuda") / side
+            yy, xx = torch.meshgrid(ar, ar, indexing="ij")
+            g = torch.stack([xx.reshape(-1), yy.reshape(-1)], 1).contiguous()
+            self._grids[side] = g
+        return g
 
     def _points(self, r):
         torch = self.torch
@@ -140,29 +154,35 @@ class Server:
             return torch.rand((r.n, 1), dtype=torch.float64, device="cuda", generator=g)
         if r.kind == "3d":
             return self.voxels
-        s = r.side
-        ar = torch.arange(s, dtype=torch.float64, device="cuda") / s
-        yy, xx = torch.meshgrid(ar, ar, indexing="ij")
-        return torch.stack([xx.reshape(-1), yy.reshape(-1)], 1).contiguous()
+        return self._grid(r.side)
 
-    def admit(self, r):
-        """Device state of a request: points, output, tables from theta."""
+    def admit(self, reqs):
+        """Device state of newly admitted requests, per kind in one go: their
+        thetas in one upload, Avees = B theta in one product and the tables of
+        all of them in one prep launch; points generated on the device
+        (shared read-only grids for 2D and 3D); outputs."""
         torch = self.torch
-        k = self.kinds[r.kind]
-        r.pts = self._points(r)
-        r.out = torch.empty_like(r.pts)
-        th = torch.from_numpy(r.theta).cuda()
-        As = (self.B[r.kind] @ th).reshape(k.space.nC, k.ne).contiguous()
-        nce = k.space.nC * k.ne
-        r.tables = torch.empty(2 * nce, dtype=torch.float64, device="cuda")
+        by = collections.defaultdict(list)
+        for r in reqs:
+            by[r.kind].append(r)
         self.ctx.bind_stream(torch.cuda.current_stream().cuda_stream)
-        self.ctx.call("cpab_prep_trels", k.dim, k.space.nC, As.data_ptr(), int(k.sign), float(self.params["dt"]),
-                      r.tables.data_ptr() + nce * 8, r.tables.data_ptr())
-        r.tiles = collections.deque((b, min(b + self.tile, r.n)) for b in range(0, r.n, self.tile))
-        r.left = len(r.tiles)
-        r.warped = None
-        if r.kind == "3d":
-            r.warped = torch.empty((VOL, VOL, VOL), dtype=torch.float32, device="cuda")
+        dt = float(self.params["dt"])
+        for kind, rs in by.items():
+            k = self.kinds[kind]
+            th = torch.from_numpy(np.stack([r.theta for r in rs])).cuda()
+            m, nce = len(rs), k.space.nC * k.ne
+            Av = self.avees(kind, th)
+            T = torch.empty((m, nce), dtype=torch.float64, device="cuda")
+            S = torch.empty((m, nce), dtype=torch.float64, device="cuda")
+            self.ctx.call("cpab_prep_trels", k.dim, m * k.space.nC, Av.data_ptr(), int(k.sign), dt, S.data_ptr(),
+                          T.data_ptr())
+            for i, r in enumerate(rs):
+                r.trels, r.As = T[i], S[i]
+                r.pts = self._points(r)
+                r.out = torch.empty_like(r.pts)
+                r.warped = torch.empty((VOL, VOL, VOL), dtype=torch.float32, device="cuda") if kind == "3d" else None
+                r.next = 0
+                r.left = -(-r.n // self.tile)
 
     def finish(self, r):
         """After the last tile: the 3D warp's remap (trilinear, calc_T_inv points)."""
@@ -178,37 +198,49 @@ class Server:
         taken = []
         by_cls = collections.OrderedDict()
         for r in active:
-            if r.tiles:
+            if r.next < r.n:
                 by_cls.setdefault(self.kinds[r.kind].cls, []).append(r)
         self.ctx.bind_stream(torch.cuda.current_stream().cuda_stream)
         p = self.params
+        tile = self.tile
         for (dim, tess), reqs in by_cls.items():
-            items, budget = [], self.max_points
+            plan, budget = [], self.max_points
             for r in reqs:
-                k = self.kinds[r.kind]
-                cnt = 0
-                while r.tiles and budget > 0:
-                    b, e = r.tiles.popleft()
-                    items.append(WorkItem(r.tables.data_ptr(), r.pts.data_ptr(), r.out.data_ptr(), b, e, *k.geo))
-                    budget -= e - b
-                    cnt += 1
-                if cnt:
-                    taken.append((r, cnt))
+                cnt = min(-(-(r.n - r.next) // tile), max(1, budget // tile))
+                b = r.next + tile * np.arange(cnt, dtype=np.int64)
+                e = np.minimum(b + tile, r.n)
+                plan.append((r, b, e))
+                r.next = int(e[-1])
+                budget -= int(e[-1] - b[0])
+                taken.append((r, cnt))
                 if budget <= 0:
                     break
-            arr = (WorkItem * len(items))(*items)
-            self.ctx.call("cpab_calc_T_items", dim, 1 if tess == "I" else 2, len(items), arr, float(p["dt"]),
-                          int(p["nTimeSteps"]), int(p["nStepsODEsolver"]), ctypes.byref(self.sched))
+            # one record array with the C layout (np.concatenate would drop its padding)
+            arr = np.empty(sum(len(b) for _, b, _ in plan), dtype=ITEM)
+            o = 0
+            for r, b, e in plan:
+                k = self.kinds[r.kind]
+                a = arr[o:o + len(b)]
+                a["trels"], a["As"] = r.trels.data_ptr(), r.As.data_ptr()
+                a["pos0"], a["out"] = r.pts.data_ptr(), r.out.data_ptr()
+                a["begin"], a["end"] = b, e
+                a["nC0"], a["nC1"], a["nC2"] = k.geo[0], k.geo[1], k.geo[2]
+                a["inc_x"], a["inc_y"], a["inc_z"] = k.geo[3], k.geo[4], k.geo[5]
+                o += len(b)
+            assert arr.dtype.itemsize == 88 and arr.flags["C_CONTIGUOUS"]
+            self.ctx.call("cpab_calc_T_items", dim, 1 if tess == "I" else 2, len(arr), arr.ctypes.data,
+                          float(p["dt"]), int(p["nTimeSteps"]), int(p["nStepsODEsolver"]), ctypes.byref(self.sched))
         # requests whose last tile is in this iteration: their remap follows
         # in stream order, before the iteration's completion event
         for r, _ in taken:
-            if not r.tiles:
+            if r.next >= r.n:
                 self.finish(r)
         return taken
 
     def run(self, requests, keep=False, realtime=True):
         """Serve `requests` (sorted by arrival).  Returns a stats dict; with
-        keep, every request keeps its result tensors in .result."""
+        keep, every request keeps its input points and result in .inputs and
+        .result."""
         torch = self.torch
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
@@ -223,14 +255,20 @@ class Server:
             t = now()
             while pending and (pending[0].arrival <= t or not realtime):
                 waiting.append(pending.popleft())
+            new = []
             while waiting and used + self.bytes_of(waiting[0]) <= self.budget:
                 r = waiting.popleft()
-                self.admit(r)
-                r.t_admit = now()
                 used += self.bytes_of(r)
+                new.append(r)
+            if new:
+                self.admit(new)
+                ta = now()
+                for r in new:
+                    r.t_admit = ta
+                active.extend(new)
                 peak = max(peak, used)
-                active.append(r)
-            if len(flights) < self.inflight and any(r.tiles for r in active):
+            can = len(flights) < self.inflight and any(r.next < r.n for r in active)
+            if can:
                 taken = self.launch(active)
                 iters += 1
                 shared += len(taken) > 1
@@ -241,18 +279,19 @@ class Server:
             while flights and flights[0][0].query():
                 ev, taken = flights.popleft()
                 progressed = True
+                tdone = now()
                 for r, cnt in taken:
                     r.left -= cnt
                     if r.left == 0:
-                        r.t_done = now()
+                        r.t_done = tdone
                         used -= self.bytes_of(r)
-                        active.remove(r)
                         if keep:
+                            r.inputs = r.pts
                             r.result = r.warped if r.kind == "3d" else r.out
-                        else:
-                            r.pts = r.out = r.tables = r.warped = None
+                        r.pts = r.out = r.trels = r.As = r.warped = None
                         done.append(r)
-            if not progressed and not (len(flights) < self.inflight and any(r.tiles for r in active)):
+                active = [r for r in active if r.left > 0]
+            if not progressed and not can:
                 if flights:
                     time.sleep(2e-5)
                 elif pending and realtime:
@@ -270,15 +309,26 @@ class Server:
                 "p99_ms": float(np.percentile(lat, 99) * 1e3), "peak_bytes": int(peak), "iterations": iters,
                 "iterations_with_several_requests": shared, "per_kind": per_kind}
 
+    def avees(self, kind, theta):
+        """Avees = B theta on the device for a (m, d) stack of thetas."""
+        torch = self.torch
+        k = self.kinds[kind]
+        D = k.space.nC * k.ne
+        Av = torch.empty((theta.shape[0], D), dtype=torch.float64, device="cuda")
+        self.ctx.bind_stream(torch.cuda.current_stream().cuda_stream)
+        self.ctx.call("cpab_theta2avees", D, k.d, int(theta.shape[0]), self.B[kind].data_ptr(),
+                      theta.contiguous().data_ptr(), Av.data_ptr())
+        return Av
+
     def alone(self, r):
         """The same request through the single-call path: update_pat +
         calc_T (+ remap3d), for comparison with its result in the mix."""
         torch = self.torch
         k = self.kinds[r.kind]
         sp = k.space
-        pts = self._points(r)
+        pts = r.inputs if getattr(r, "inputs", None) is not None else self._points(r)
         out = torch.empty_like(pts)
-        sp.update_pat((self.B[r.kind] @ torch.from_numpy(r.theta).cuda()).cpu().numpy())
+        sp.update_pat(self.avees(r.kind, torch.from_numpy(r.theta).cuda().reshape(1, -1))[0].cpu().numpy())
         sp._calc_T(pts=pts, mysign=k.sign, out=out, **self.params)
         if r.kind != "3d":
             return out

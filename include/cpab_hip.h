@@ -198,17 +198,25 @@ int cpab_batch_calc_T(cpab_batch *b, const double *tables, const double *pos0, d
 int cpab_batch_destroy(cpab_batch *b);
 
 /* One tile of a served request for cpab_calc_T_items: the request's own
- * tables (Trels then signed As, as cpab_prep_trels writes them when
- * signedAs_out = Trels_out + nC*dim*(dim+1)), points and output, the point
- * range [begin, end) of this tile, and the request's tessellation. */
+ * tables (Trels and signed As of cpab_prep_trels, (nC, dim(dim+1)) each),
+ * points and output, the point range [begin, end) of this tile, and the
+ * request's tessellation. */
 typedef struct {
-  const double *tables;
+  const double *trels;
+  const double *As;
   const double *pos0;
   double *out;
   int64_t begin, end;
   int nC0, nC1, nC2;
   double inc_x, inc_y, inc_z;
 } cpab_work_item;
+
+/* Avees = B theta for m thetas at once: CpaSpace.theta2Avees
+ * (cpab/cpaNd/_CpaSpace.py:366-376).  B (D, d) row-major, theta (m, d),
+ * Av (m, D).  Each row is summed in index order, so a theta's Avees do not
+ * depend on the rest of the batch. */
+int cpab_theta2avees(cpab_ctx *ctx, int D, int d, int m, const double *B, const double *theta,
+                     double *Av);
 
 /* calc_T over a list of tiles of any requests of one dim and tess (no
  * reference counterpart: the serving loop's continuous batching), one

@@ -46,3 +46,23 @@ def test_budget_is_respected(kinds):
     say("budget run", st)
     assert st["requests"] == len(reqs)
     assert st["peak_bytes"] <= budget
+
+
+def test_theta2avees_is_batch_independent(kinds):
+    """Avees = B theta on the device: the same bits for a theta alone and
+    inside a stack, and each row the index-order sum."""
+    import torch
+    from cpabdiffeo_amd import serving
+    srv = serving.Server(kinds, n_volumes=1)
+    k = kinds["3d"]
+    rng = np.random.default_rng(0)
+    th = rng.standard_normal((5, k.d))
+    stack = srv.avees("3d", torch.from_numpy(th).cuda()).cpu().numpy()
+    one = srv.avees("3d", torch.from_numpy(th[2:3]).cuda()).cpu().numpy()
+    assert np.array_equal(stack[2], one[0])
+    B = k.space.B
+    for r in (0, 17, B.shape[0] - 1):
+        acc = 0.0
+        for j in range(k.d):
+            acc += B[r, j] * th[4, j]
+        assert stack[4, r] == acc
