@@ -32,10 +32,10 @@ class BatchSched(ctypes.Structure):
         return (self.tile, self.block, self.chunk)
 
 
-# Candidate schedules the autotuner times: tile sizes from one wave's worth of
-# refills to whole medium instances, blocks of 4 to 16 waves, 1 to 8 tiles
-# per queue grab (larger grabs reuse an instance's tables in LDS longer).
-CANDIDATES = [(t, b, c) for t in (4096, 16384, 65536) for b in (256, 1024) for c in (1, 4)]
+# Candidate schedules the autotuner times: tiles from one point per lane of a
+# 1 024-thread block to whole medium instances, blocks of 4 to 16 waves, 1 to
+# 8 tiles per queue grab (larger grabs reuse an instance's tables in LDS longer).
+CANDIDATES = [(t, b, c) for t in (1024, 4096, 16384, 65536) for b in (256, 1024) for c in (1, 8)]
 
 _TUNED = {}  # signature -> best schedule key, per process
 
