@@ -14,9 +14,11 @@ Avees -> expm tables -> calc_T_inv of the 65 536 pixel points (t = 1,
 hipGraphs.  After W warmup steps the timed region is the K-step chain from
 theta = 0 with its counters reset, so the timed work does not depend on W.
 
-Each launch evaluates the next 16 proposals on the all-accepted path and
+Each launch evaluates the next K proposals on the all-accepted path and
 decides them in order (speculative prefetch): the chain is the same bit
 for bit as one step per launch, which is timed too and reported beside it.
+K is tuned on the workload itself before the timed run (each candidate
+runs the same chain from theta = 0).
 
 Prints one JSON line: proposals/s, the live roofline of the dominant kernel
 (algorithmic FP64 FLOPs of the proposals each launch evaluates, over the
@@ -44,17 +46,18 @@ F_REMAP_SSD = 16         # bilinear sample (13) + squared error accumulation (3)
 # midpoint 6, g2 2, L^T g2 6, g1 4, lambda update 10, dJ/dA accumulation 20
 F_ADJ_SUB = 71
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
-# HBM bytes per launch of the step kernel at 16 proposals per launch, from the
-# FETCH_SIZE / WRITE_SIZE counter runs scaled by a plain copy (committed summary).
+# HBM bytes of the step kernel per evaluated proposal, from the FETCH_SIZE /
+# WRITE_SIZE counter runs (16 proposals per launch) scaled by a plain copy
+# (committed summary); a launch's traffic is that times the proposals it evaluates.
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_mcmc_step_registration256_spec16.md")
 
 
-def measured_traffic(spec_depth):
+def measured_traffic(proposals_per_launch):
     import re
-    if spec_depth != 16 or not os.path.exists(TRAFFIC_SUMMARY):
+    if not os.path.exists(TRAFFIC_SUMMARY):
         return None
-    m = re.search(r"traffic_bytes_per_launch = (\d+)", open(TRAFFIC_SUMMARY).read())
-    return int(m.group(1)) if m else None
+    m = re.search(r"traffic_bytes_per_proposal = (\d+)", open(TRAFFIC_SUMMARY).read())
+    return int(m.group(1)) * proposals_per_launch if m else None
 
 
 def log(*a):
@@ -320,16 +323,34 @@ def main():
     use_graph = not args.no_graph
     ctx = Context.get(0)
 
-    def chain(spec_depth, warmup, steps, timed_events):
-        """warmup steps, then the timed chain: theta = 0, counters reset, `steps` steps."""
-        mh = DeviceMetropolis(problem, nChains=1, scale=0.01, seed=0, count_fallbacks=True, spec_depth=spec_depth)
-        mh.set_theta(np.zeros(mh.d))
-        mh.run(warmup, use_graph=use_graph)
-        torch.cuda.synchronize()
+    def reset(mh):
         for t in (mh.step_counter, mh.n_accepted, mh.n_fallback, mh.stats):
             t.zero_()
         mh.set_theta(np.zeros(mh.d))
         torch.cuda.synchronize()
+
+    def chain(spec_depth, warmup, steps, timed_events, tune=()):
+        """warmup steps; with `tune`, the workload itself (steps from theta = 0)
+        at each candidate speculation depth, keeping the fastest; then the
+        timed chain: theta = 0, counters reset, `steps` steps."""
+        mh = DeviceMetropolis(problem, nChains=1, scale=0.01, seed=0, count_fallbacks=True, spec_depth=spec_depth)
+        mh.set_theta(np.zeros(mh.d))
+        mh.run(warmup, use_graph=use_graph)
+        torch.cuda.synchronize()
+        rates = {}
+        for K in tune:
+            try:
+                mh.set_spec_depth(K)
+            except ValueError:
+                continue
+            reset(mh)
+            t0 = time.perf_counter()
+            mh.run(steps, use_graph=use_graph)
+            torch.cuda.synchronize()
+            rates[K] = steps / (time.perf_counter() - t0)
+        if rates:
+            mh.set_spec_depth(max(rates, key=rates.get))
+        reset(mh)
         if timed_events:
             ctx.prof_enable(True)
             ctx.prof_read()
@@ -341,12 +362,13 @@ def main():
         ctx.prof_enable(False)
         out = dict(elapsed=elapsed, events=ev, stats=mh.stats.cpu().numpy()[0].copy(),
                    nacc=int(mh.n_accepted.cpu().numpy()[0]), nfb=int(mh.n_fallback.cpu().numpy()[0]),
-                   ll=float(mh.ll.cpu().numpy()[0]), theta=mh.theta.cpu().numpy()[0].copy(), K=mh.spec_depth)
+                   ll=float(mh.ll.cpu().numpy()[0]), theta=mh.theta.cpu().numpy()[0].copy(), K=mh.spec_depth,
+                   tune=rates)
         mh.close()
         return out
 
     log("bench: warmup %d steps, then the timed %d-step chain (speculative prefetch)" % (args.warmup, args.steps))
-    spec = chain(0, args.warmup, args.steps, True)
+    spec = chain(0, args.warmup, args.steps, True, tune=(8, 12, 16, 20, 24, 32))
     seq = None
     if not args.no_seq:
         log("bench: the same chain one step per launch")
@@ -414,13 +436,14 @@ def main():
                                "nC=64 tess I, calc_T_inv t=1 100x10 fp64 + bilinear remap + SSD sigma=0.02"
                                % args.steps,
                    "points_per_proposal": npts, "graph_replay": use_graph, "spec_depth": spec["K"],
+                   "spec_depth_autotune_proposals_per_sec": {str(k): v for k, v in spec["tune"].items()},
                    "accept_ratio": nacc / float(args.steps)},
         "points_per_sec": proposals_per_sec * npts,
         "one_step_per_launch": None if seq is None else {
             "proposals_per_sec": args.steps / seq["elapsed"], "speedup_of_speculation": seq["elapsed"] / elapsed,
             "same_chain_bit_for_bit": same_chain},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(spec["K"]),
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(st[0] / launches),
                      "kernel": "k_mcmc_step (K proposals: calc_T_inv + remap + SSD; accept; arm next)",
                      "kernel_us": kern_s * 1e6, "launches": launches,
                      "proposals_evaluated_per_launch": st[0] / launches,

@@ -1063,6 +1063,7 @@ constexpr int kMaxSpec = 32;
 struct cpab_reg2d {
   cpab_ctx* ctx = nullptr;
   McmcDesc D{};
+  int K_cap = 1;  // slots the scratch holds (cpab_reg2d_set_spec_depth may go up to it)
   bool pow2 = true;
   double* scratch = nullptr;  // theta_prop | ll_prop | tables | partials | stop | done
   hipStream_t cap = nullptr;
@@ -1173,6 +1174,14 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   }
   // the epilogue's workspace must fit in LDS beside nothing else
   while (K > 1 && (size_t)mcmc_finish_doubles(K, q.d) * sizeof(double) > (size_t)ctx->max_lds) K--;
+  // room for twice the library's own choice, for later tuning of the depth
+  int K_cap = K;
+  {
+    const int64_t k2 = 2 * kSpecPoints / (nPts * (int64_t)q.nChains);
+    const int auto2 = k2 < 1 ? 1 : (k2 > kMaxSpec ? kMaxSpec : (int)k2);
+    if (auto2 > K_cap) K_cap = auto2;
+    while (K_cap > K && (size_t)mcmc_finish_doubles(K_cap, q.d) * sizeof(double) > (size_t)ctx->max_lds) K_cap--;
+  }
   if ((size_t)(2 * q.nC * 6) * sizeof(double) > (size_t)ctx->max_lds ||
       (size_t)mcmc_finish_doubles(K, q.d) * sizeof(double) > (size_t)ctx->max_lds)
     return fail(ctx, CPAB_ERR_UNSUPPORTED, "per-cell tables or proposal workspace exceed LDS");
@@ -1221,8 +1230,9 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
     delete r;
     return fail(ctx, CPAB_ERR_ARG, "image too large for the launch grid");
   }
+  r->K_cap = K_cap;
   // scratch, zeroed once: theta_prop | ll_prop | tables | partial | partial_nfb | stop | done
-  const size_t nck = (size_t)q.nChains * K;
+  const size_t nck = (size_t)q.nChains * K_cap;
   const size_t n = nck * q.d + q.nChains + nck * 2 * q.nC * 6 + 2 * nck * D.nBlocks + q.nChains + q.nChains;
   hipError_t e = hipMalloc(&r->scratch, n * sizeof(double));
   if (e == hipSuccess) e = hipMemset(r->scratch, 0, n * sizeof(double));
@@ -1298,6 +1308,28 @@ int cpab_reg2d_last_proposal(cpab_reg2d* r, double* theta_prop, double* ll_prop)
 int cpab_reg2d_spec_depth(cpab_reg2d* r, int* K) {
   if (!r || !K) return CPAB_ERR_ARG;
   *K = r->D.K;
+  return CPAB_OK;
+}
+
+int cpab_reg2d_set_spec_depth(cpab_reg2d* r, int K) {
+  if (!r) return CPAB_ERR_ARG;
+  cpab_ctx* ctx = r->ctx;
+  McmcDesc& D = r->D;
+  if (K < 1 || K > r->K_cap) return fail(ctx, CPAB_ERR_ARG, "spec_depth must be in [1, %d] for this chain set", r->K_cap);
+  if (K == D.K) return CPAB_OK;
+  if ((int64_t)D.nBlocks * K > 0x7fffffff) return fail(ctx, CPAB_ERR_ARG, "image too large for the launch grid");
+  // the captured launches carry the old descriptor
+  HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+  if (r->exec) (void)hipGraphExecDestroy(r->exec);
+  if (r->graph) (void)hipGraphDestroy(r->graph);
+  r->exec = nullptr;
+  r->graph = nullptr;
+  D.K = K;
+  // re-arm the proposals of the current step at the new depth: the same
+  // numbers the chain would use, so the chain is unchanged
+  const size_t sh = (size_t)mcmc_arm_doubles(D.K, D.d) * sizeof(double);
+  hipLaunchKernelGGL(k_mcmc_prep, dim3(D.nChains), dim3(256), sh, ctx->stream, D, 1);
+  HIP_OK(ctx, hipGetLastError());
   return CPAB_OK;
 }
 

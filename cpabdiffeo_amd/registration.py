@@ -165,6 +165,33 @@ class DeviceMetropolis:
         self.nSteps += N
         self.record["runs"].append({"N": N, "enqueue_s": time.perf_counter() - t0})
 
+    def set_spec_depth(self, K):
+        """Proposals per chain and launch from now on (the chain is unchanged)."""
+        self._bind()
+        check(lib().cpab_reg2d_set_spec_depth(self._h, int(K)), self._ctx.handle)
+        self.spec_depth = int(K)
+
+    def autotune_spec(self, candidates=(8, 12, 16, 20, 24, 32), steps=192, use_graph=True):
+        """Time `steps` Metropolis steps of these chains at each candidate
+        speculation depth (the steps are real steps of the chains) and keep
+        the fastest.  Returns {depth: steps per second}."""
+        torch = require_gpu()
+        rates = {}
+        for K in candidates:
+            try:
+                self.set_spec_depth(K)
+            except ValueError:
+                continue  # beyond what this chain set was sized for
+            self.run(16, use_graph=use_graph)  # capture and warm at this depth
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            self.run(steps, use_graph=use_graph)
+            torch.cuda.synchronize()
+            rates[K] = steps / (time.perf_counter() - t0)
+        best = max(rates, key=rates.get)
+        self.set_spec_depth(best)
+        return rates
+
     def last_proposal(self):
         """(theta' the next step will evaluate, ll of the last evaluated proposal)."""
         torch = require_gpu()

@@ -294,6 +294,11 @@ int cpab_reg2d_run(cpab_reg2d *r, int nSteps, int use_graph);
 int cpab_reg2d_last_proposal(cpab_reg2d *r, double *theta_prop, double *ll_prop);
 /* The speculation depth in use (spec_depth as resolved at create). */
 int cpab_reg2d_spec_depth(cpab_reg2d *r, int *K);
+/* Change the speculation depth between runs (up to twice the library's own
+ * choice at create, or the depth asked for if larger); the armed proposals
+ * are recomputed for the new depth, so the chain is unchanged.  Call after
+ * cpab_reg2d_init.  Synchronises the context's stream. */
+int cpab_reg2d_set_spec_depth(cpab_reg2d *r, int K);
 int cpab_reg2d_destroy(cpab_reg2d *r);
 
 /* Kernel timing with HIP events on the context's stream (no reference

@@ -124,3 +124,24 @@ def test_speculation_is_the_same_chain(problem, gpu, nChains, spec, graph):
     assert (st[:, 3] == 60).all()  # steps decided
     assert (st[:, 0] >= 60).all()  # proposals evaluated
     assert 0 < a[2].min() and a[2].max() < 60  # accepts and rejects both happened
+
+
+def test_changing_the_speculation_depth_keeps_the_chain(problem, gpu):
+    import torch
+    from cpabdiffeo_amd.registration import DeviceMetropolis
+    say("speculation depth changed between runs")
+    p = problem
+    out = []
+    for plan in ([(1, 60)], [(0, 11), (5, 7), (32, 20), (1, 2), (16, 20)]):
+        mh = DeviceMetropolis(p, nChains=1, scale=0.05, seed=4, trace_len=60, count_fallbacks=True,
+                              spec_depth=plan[0][0])
+        mh.set_theta(np.zeros(mh.d))
+        for K, n in plan:
+            if K:
+                mh.set_spec_depth(K)
+            mh.run(n)
+        torch.cuda.synchronize()
+        out.append([x.cpu().numpy().copy() for x in (mh.theta, mh.trace, mh.n_accepted, mh.n_fallback, mh.step_counter)])
+        mh.close()
+    for i in range(5):
+        assert np.array_equal(out[0][i], out[1][i]), i
