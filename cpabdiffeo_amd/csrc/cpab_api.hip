@@ -1227,7 +1227,13 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   // hiding needs the waves); with many (batched chains) 2 or 4 pixels per
   // lane with refill keep lanes busy past their slowest pixel (measured on
   // MI355X: 1 chain x 20 slots best at 64, 256 chains at 256, +13 %).
-  {
+  if (q.pts_per_wave != 0) {
+    if (q.pts_per_wave < 64 || q.pts_per_wave > 4096 || (q.pts_per_wave % 64) != 0) {
+      delete r;
+      return fail(ctx, CPAB_ERR_ARG, "pts_per_wave must be a multiple of 64 in [64, 4096] or 0");
+    }
+    D.ptsPerWave = q.pts_per_wave;
+  } else {
     const int64_t waves64 = (int64_t)q.nChains * K * ((nPts + 63) / 64);
     const int64_t full = (int64_t)ctx->n_cu * 4 * 8;
     D.ptsPerWave = waves64 >= 32 * full ? 256 : (waves64 >= 8 * full ? 128 : 64);

@@ -38,7 +38,7 @@ class RegDesc(ctypes.Structure):
                 ("src", ctypes.c_void_p), ("dst", ctypes.c_void_p), ("theta", ctypes.c_void_p),
                 ("ll", ctypes.c_void_p), ("n_accepted", ctypes.c_void_p), ("step", ctypes.c_void_p),
                 ("trace", ctypes.c_void_p), ("trace_len", ctypes.c_int), ("n_fallback", ctypes.c_void_p),
-                ("spec_depth", ctypes.c_int), ("stats", ctypes.c_void_p)]
+                ("spec_depth", ctypes.c_int), ("stats", ctypes.c_void_p), ("pts_per_wave", ctypes.c_int)]
 
 
 def pixel_basis(space_unit, n):
@@ -87,10 +87,13 @@ class DeviceMetropolis:
     The API follows Metropolis (set_theta / step / run / record); theta has
     shape (nChains, d).  spec_depth: proposals evaluated per chain and launch
     (1: one step per launch; K > 1: speculative prefetch of the next K steps,
-    the same chain bit for bit; 0: chosen by the library)."""
+    the same chain bit for bit; 0: chosen by the library).  pts_per_wave:
+    pixels per wave (0: by the launch size); chains run with the same value
+    give the same bits alone and in a batch."""
 
     def __init__(self, problem, nChains=1, scale=0.01, diagonal=False, use_ave_ll=True,
-                 inverted_acceptance=False, seed=0, trace_len=0, count_fallbacks=False, spec_depth=1):
+                 inverted_acceptance=False, seed=0, trace_len=0, count_fallbacks=False, spec_depth=1,
+                 pts_per_wave=0):
         torch = require_gpu()
         self.problem = problem
         self.nChains = nChains
@@ -128,7 +131,7 @@ class DeviceMetropolis:
                        self._dst.data_ptr(), self.theta.data_ptr(), self.ll.data_ptr(), self.n_accepted.data_ptr(),
                        self.step_counter.data_ptr(), self.trace.data_ptr() if trace_len else None, int(trace_len),
                        self.n_fallback.data_ptr() if count_fallbacks else None, int(spec_depth),
-                       self.stats.data_ptr())
+                       self.stats.data_ptr(), int(pts_per_wave))
         self._ctx = Context.get(0)
         self._ctx.bind_stream(torch.cuda.current_stream().cuda_stream)
         h = ctypes.c_void_p()

@@ -145,3 +145,19 @@ def test_changing_the_speculation_depth_keeps_the_chain(problem, gpu):
         mh.close()
     for i in range(5):
         assert np.array_equal(out[0][i], out[1][i]), i
+
+
+def test_chain_alone_equals_chain_in_a_batch_at_equal_pixels_per_wave(problem, gpu):
+    import torch
+    from cpabdiffeo_amd.registration import DeviceMetropolis
+    say("alone vs batch at 256 pixels per wave")
+    p = problem
+    res = []
+    for nChains in (1, 5):
+        mh = DeviceMetropolis(p, nChains=nChains, scale=0.05, seed=8, trace_len=16, pts_per_wave=256)
+        mh.set_theta(np.zeros(mh.d))
+        mh.run(16)
+        torch.cuda.synchronize()
+        res.append((mh.theta.cpu().numpy()[0].copy(), mh.trace.cpu().numpy()[0].copy()))
+        mh.close()
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
