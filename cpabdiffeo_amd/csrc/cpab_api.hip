@@ -1091,11 +1091,11 @@ int reg2d_launch_step(cpab_reg2d* r, hipStream_t s, int init) {
     if (rc) return rc;
   }
   if (D.tess == TESS_I) {
-    if (r->pow2) hipLaunchKernelGGL((k_mcmc_step<TESS_I, true>), grid, dim3(256), sh, s, D, init);
-    else hipLaunchKernelGGL((k_mcmc_step<TESS_I, false>), grid, dim3(256), sh, s, D, init);
+    if (r->pow2) hipLaunchKernelGGL((k_mcmc_step<TESS_I, true>), grid, dim3(MCMC_BLOCK), sh, s, D, init);
+    else hipLaunchKernelGGL((k_mcmc_step<TESS_I, false>), grid, dim3(MCMC_BLOCK), sh, s, D, init);
   } else {
-    if (r->pow2) hipLaunchKernelGGL((k_mcmc_step<TESS_II, true>), grid, dim3(256), sh, s, D, init);
-    else hipLaunchKernelGGL((k_mcmc_step<TESS_II, false>), grid, dim3(256), sh, s, D, init);
+    if (r->pow2) hipLaunchKernelGGL((k_mcmc_step<TESS_II, true>), grid, dim3(MCMC_BLOCK), sh, s, D, init);
+    else hipLaunchKernelGGL((k_mcmc_step<TESS_II, false>), grid, dim3(MCMC_BLOCK), sh, s, D, init);
   }
   if (timed) {
     int rc = prof_end(r->ctx, a, b);
@@ -1239,7 +1239,7 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
     D.ptsPerWave = waves64 >= 32 * full ? 256 : (waves64 >= 8 * full ? 128 : 64);
   }
   const int64_t waves = (nPts + D.ptsPerWave - 1) / D.ptsPerWave;
-  D.nBlocks = (int)((waves + 3) / 4);
+  D.nBlocks = (int)((waves + MCMC_BLOCK / 64 - 1) / (MCMC_BLOCK / 64));
   if ((int64_t)D.nBlocks * K > 0x7fffffff) {
     delete r;
     return fail(ctx, CPAB_ERR_ARG, "image too large for the launch grid");
@@ -1271,7 +1271,7 @@ int cpab_reg2d_init(cpab_reg2d* r) {
   if (!r) return CPAB_ERR_ARG;
   McmcDesc& D = r->D;
   const size_t sh = (size_t)mcmc_arm_doubles(D.K, D.d) * sizeof(double);
-  hipLaunchKernelGGL(k_mcmc_prep, dim3(D.nChains), dim3(256), sh, r->ctx->stream, D, 0);
+  hipLaunchKernelGGL(k_mcmc_prep, dim3(D.nChains), dim3(MCMC_BLOCK), sh, r->ctx->stream, D, 0);
   HIP_OK(r->ctx, hipGetLastError());
   return reg2d_launch_step(r, r->ctx->stream, 1);
 }
@@ -1342,7 +1342,7 @@ int cpab_reg2d_set_spec_depth(cpab_reg2d* r, int K) {
   // re-arm the proposals of the current step at the new depth: the same
   // numbers the chain would use, so the chain is unchanged
   const size_t sh = (size_t)mcmc_arm_doubles(D.K, D.d) * sizeof(double);
-  hipLaunchKernelGGL(k_mcmc_prep, dim3(D.nChains), dim3(256), sh, ctx->stream, D, 1);
+  hipLaunchKernelGGL(k_mcmc_prep, dim3(D.nChains), dim3(MCMC_BLOCK), sh, ctx->stream, D, 1);
   HIP_OK(ctx, hipGetLastError());
   return CPAB_OK;
 }
