@@ -1064,6 +1064,7 @@ struct cpab_reg2d {
   cpab_ctx* ctx = nullptr;
   McmcDesc D{};
   int K_cap = 1;  // slots the scratch holds (cpab_reg2d_set_spec_depth may go up to it)
+  int block = MCMC_BLOCK;  // threads per block of the step kernel
   bool pow2 = true;
   double* scratch = nullptr;  // theta_prop | ll_prop | tables | partials | stop | done
   hipStream_t cap = nullptr;
@@ -1090,13 +1091,19 @@ int reg2d_launch_step(cpab_reg2d* r, hipStream_t s, int init) {
     int rc = prof_begin(r->ctx, &a, &b);
     if (rc) return rc;
   }
+#define STEP_LAUNCH(T_, P_)                                                                            \
+  do {                                                                                                 \
+    if (r->block == 64) hipLaunchKernelGGL((k_mcmc_step<T_, P_, 64>), grid, dim3(64), sh, s, D, init); \
+    else hipLaunchKernelGGL((k_mcmc_step<T_, P_, 256>), grid, dim3(256), sh, s, D, init);           \
+  } while (0)
   if (D.tess == TESS_I) {
-    if (r->pow2) hipLaunchKernelGGL((k_mcmc_step<TESS_I, true>), grid, dim3(MCMC_BLOCK), sh, s, D, init);
-    else hipLaunchKernelGGL((k_mcmc_step<TESS_I, false>), grid, dim3(MCMC_BLOCK), sh, s, D, init);
+    if (r->pow2) STEP_LAUNCH(TESS_I, true);
+    else STEP_LAUNCH(TESS_I, false);
   } else {
-    if (r->pow2) hipLaunchKernelGGL((k_mcmc_step<TESS_II, true>), grid, dim3(MCMC_BLOCK), sh, s, D, init);
-    else hipLaunchKernelGGL((k_mcmc_step<TESS_II, false>), grid, dim3(MCMC_BLOCK), sh, s, D, init);
+    if (r->pow2) STEP_LAUNCH(TESS_II, true);
+    else STEP_LAUNCH(TESS_II, false);
   }
+#undef STEP_LAUNCH
   if (timed) {
     int rc = prof_end(r->ctx, a, b);
     if (rc) return rc;
@@ -1237,9 +1244,11 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
     const int64_t waves64 = (int64_t)q.nChains * K * ((nPts + 63) / 64);
     const int64_t full = (int64_t)ctx->n_cu * 4 * 8;
     D.ptsPerWave = waves64 >= 32 * full ? 256 : (waves64 >= 8 * full ? 128 : 64);
+    // many short-armed chains: one-wave blocks release CUs sooner
+    if (K == 1 && waves64 >= 8 * full) r->block = 64;
   }
   const int64_t waves = (nPts + D.ptsPerWave - 1) / D.ptsPerWave;
-  D.nBlocks = (int)((waves + MCMC_BLOCK / 64 - 1) / (MCMC_BLOCK / 64));
+  D.nBlocks = (int)((waves + r->block / 64 - 1) / (r->block / 64));
   if ((int64_t)D.nBlocks * K > 0x7fffffff) {
     delete r;
     return fail(ctx, CPAB_ERR_ARG, "image too large for the launch grid");
@@ -1271,7 +1280,7 @@ int cpab_reg2d_init(cpab_reg2d* r) {
   if (!r) return CPAB_ERR_ARG;
   McmcDesc& D = r->D;
   const size_t sh = (size_t)mcmc_arm_doubles(D.K, D.d) * sizeof(double);
-  hipLaunchKernelGGL(k_mcmc_prep, dim3(D.nChains), dim3(MCMC_BLOCK), sh, r->ctx->stream, D, 0);
+  hipLaunchKernelGGL(k_mcmc_prep, dim3(D.nChains), dim3(256), sh, r->ctx->stream, D, 0);
   HIP_OK(r->ctx, hipGetLastError());
   return reg2d_launch_step(r, r->ctx->stream, 1);
 }
@@ -1342,7 +1351,7 @@ int cpab_reg2d_set_spec_depth(cpab_reg2d* r, int K) {
   // re-arm the proposals of the current step at the new depth: the same
   // numbers the chain would use, so the chain is unchanged
   const size_t sh = (size_t)mcmc_arm_doubles(D.K, D.d) * sizeof(double);
-  hipLaunchKernelGGL(k_mcmc_prep, dim3(D.nChains), dim3(MCMC_BLOCK), sh, ctx->stream, D, 1);
+  hipLaunchKernelGGL(k_mcmc_prep, dim3(D.nChains), dim3(256), sh, ctx->stream, D, 1);
   HIP_OK(ctx, hipGetLastError());
   return CPAB_OK;
 }

@@ -68,14 +68,12 @@ __host__ __device__ inline double u53(uint32_t a, uint32_t b) {
 
 constexpr uint32_t SLOT_ACCEPT = 0x80000000u;
 
-// Threads per block of the registration kernels.  256 (4 waves): a block
-// keeps its CU resources until its slowest wave ends, and the slowest pixels
-// (the domain corner) make that wait long; measured on MI355X, 1 024-thread
-// blocks are 20 % slower for one chain and 13 % for 256 chains.
-#ifndef CPAB_MCMC_BLOCK
-#define CPAB_MCMC_BLOCK 256
-#endif
-constexpr int MCMC_BLOCK = CPAB_MCMC_BLOCK;
+// Threads per block of the registration step: 256 (4 waves) or 64.  A block
+// keeps its CU resources until its slowest wave ends, so small blocks free
+// them sooner, but the last block of a chain arms the next proposals with
+// its threads.  Measured on MI355X: one chain with speculation is fastest at
+// 256 (64: -50 %, 1 024: -20 %), 256 chains one step per launch at 64 (+12 %).
+constexpr int MCMC_BLOCK = 256;
 
 struct McmcDesc {
   int nChains, d, nC, nC0, nC1, tess;
@@ -190,7 +188,7 @@ __device__ inline void mcmc_arm(const McmcDesc& D, int ch, int64_t step, int wit
 }
 
 // One block per chain: arm the proposals of the chain's current step.
-__global__ void __launch_bounds__(MCMC_BLOCK) k_mcmc_prep(McmcDesc D, int with_noise) {
+__global__ void __launch_bounds__(256) k_mcmc_prep(McmcDesc D, int with_noise) {
   extern __shared__ double w[];
   const int ch = blockIdx.x;
   mcmc_arm(D, ch, D.step[ch], with_noise, D.theta + (int64_t)ch * D.d, w, w + D.K * D.d);
@@ -302,10 +300,10 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
 // the current state.  A chain whose run has ended (step >= stop) is left as
 // it is.  Every block of a live chain, evaluating or not, counts its arrival,
 // so the last arrival is the only block that touches the chain's state.
-template <int TESS, bool POW2>
-__global__ void __launch_bounds__(MCMC_BLOCK) k_mcmc_step(McmcDesc D, int init) {
+template <int TESS, bool POW2, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
   extern __shared__ double lds[];
-  __shared__ double red[MCMC_BLOCK / 64], rednf[MCMC_BLOCK / 64];
+  __shared__ double red[BLOCK / 64], rednf[BLOCK / 64];
   __shared__ int last;
   __shared__ int fin[2];
   const int ch = blockIdx.y;
