@@ -49,7 +49,7 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
 # HBM bytes of the step kernel per evaluated proposal, from the FETCH_SIZE /
 # WRITE_SIZE counter runs (16 proposals per launch) scaled by a plain copy
 # (committed summary); a launch's traffic is that times the proposals it evaluates.
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_mcmc_step_registration256_spec16.md")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_mcmc_step_registration256_spec.md")
 
 
 def measured_traffic(proposals_per_launch):

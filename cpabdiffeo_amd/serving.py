@@ -313,6 +313,28 @@ class Server:
                 "p99_ms": float(np.percentile(lat, 99) * 1e3), "peak_bytes": int(peak), "iterations": iters,
                 "iterations_with_several_requests": shared, "per_kind": per_kind}
 
+    def autotune(self, requests, candidates=((1024, 1024, 1 << 21), (2048, 1024, 1 << 21), (4096, 1024, 1 << 21),
+                                             (4096, 1024, 1 << 22)), budget_p99=None):
+        """Serve the same sample of requests (in real time) under each
+        candidate schedule (tile, block, max_points) and keep the one with the
+        lowest p99 latency; with `budget_p99` (seconds), the highest capacity
+        among those within it.  Returns {schedule: stats}."""
+        import copy
+        res = {}
+        for tile, block, mp in candidates:
+            self.tile, self.sched, self.max_points = int(tile), Sched(int(tile), int(block), 1), int(mp)
+            st = self.run([copy.copy(r) for r in requests])
+            cap = self.run([copy.copy(r) for r in requests], realtime=False)
+            st["capacity_points_per_sec"] = cap["points_per_sec"]
+            res[(tile, block, mp)] = st
+        ok = [k for k, v in res.items() if budget_p99 is None or v["p99_ms"] * 1e-3 <= budget_p99]
+        if budget_p99 is None or not ok:
+            best = min(res, key=lambda k: res[k]["p99_ms"])
+        else:
+            best = max(ok, key=lambda k: res[k]["capacity_points_per_sec"])
+        self.tile, self.sched, self.max_points = best[0], Sched(best[0], best[1], 1), best[2]
+        return res
+
     def avees(self, kind, theta):
         """Avees = B theta on the device for a (m, d) stack of thetas."""
         torch = self.torch

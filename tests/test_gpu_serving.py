@@ -66,3 +66,13 @@ def test_theta2avees_is_batch_independent(kinds):
         for j in range(k.d):
             acc += B[r, j] * th[4, j]
         assert stack[4, r] == acc
+
+
+def test_autotune_keeps_the_lowest_p99_schedule(kinds):
+    from cpabdiffeo_amd import serving
+    reqs = serving.make_mix(30, kinds, rate=3000.0, seed=4, n1d=(1e3, 2e4), sides=(32, 64))
+    srv = serving.Server(kinds, n_volumes=1, params=dict(dt=0.02, nTimeSteps=20, nStepsODEsolver=10))
+    res = srv.autotune(reqs, candidates=((1024, 1024, 1 << 16), (4096, 256, 1 << 15)))
+    best = min(res, key=lambda k: res[k]["p99_ms"])
+    assert (srv.tile, srv.sched.block, srv.max_points) == best
+    assert all(v["requests"] == len(reqs) for v in res.values())
