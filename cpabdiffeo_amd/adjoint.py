@@ -142,11 +142,6 @@ def transform_simple(theta, pts, B, space, params, sign=1):
     return _FN[1].apply(theta, pts, B, space, params, sign)
 
 
-def fd_directional(loss_fn, theta, v, h=1e-6):
-    """Central finite difference of loss_fn along v (FP64)."""
-    return (loss_fn(theta + h * v) - loss_fn(theta - h * v)) / (2.0 * h)
-
-
 def ssd_cotangent(src, dst, pts):
     """Torch FP64 restatement, for self-checks, of what the SSD kernel
     differentiates: w = the bilinear sample of src (H, W) at pts (N, 2)
