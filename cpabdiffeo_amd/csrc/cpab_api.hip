@@ -1066,7 +1066,7 @@ struct cpab_reg2d {
   int K_cap = 1;  // slots the scratch holds (cpab_reg2d_set_spec_depth may go up to it)
   int block = MCMC_BLOCK;  // threads per block of the step kernel
   bool pow2 = true;
-  double* scratch = nullptr;  // theta_prop | ll_prop | tables | partials | stop | done
+  double* scratch = nullptr;  // theta_prop | ll_prop | tables | partials | Bt | done_slot | stop | done
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1080,11 +1080,21 @@ size_t reg2d_lds(const McmcDesc& D) {
   return (size_t)(a > b ? a : b) * sizeof(double);
 }
 
+// Tables of the first nk armed proposals of every chain.
+void reg2d_launch_tables(cpab_reg2d* r, hipStream_t s, int nk) {
+  const McmcDesc& D = r->D;
+  const size_t sh = (size_t)mcmc_tables_doubles(D.nC, D.d) * sizeof(double);
+  const int rows = 6 * D.nC;
+  const unsigned threads = rows >= 1024 ? 1024u : (unsigned)((rows + 63) / 64 * 64);
+  hipLaunchKernelGGL(k_mcmc_tables, dim3((unsigned)nk, D.nChains), dim3(threads), sh, s, D);
+}
+
+// One step launch and the tables of the proposals it arms.
 int reg2d_launch_step(cpab_reg2d* r, hipStream_t s, int init) {
   McmcDesc& D = r->D;
   const size_t sh = reg2d_lds(D);
   const dim3 grid((unsigned)(D.nBlocks * D.K), D.nChains);
-  // events bracket the step kernel when profiling (not inside a capture)
+  // events bracket the two kernels when profiling (not inside a capture)
   const bool timed = r->ctx->prof && s == r->ctx->stream;
   hipEvent_t a = nullptr, b = nullptr;
   if (timed) {
@@ -1104,6 +1114,7 @@ int reg2d_launch_step(cpab_reg2d* r, hipStream_t s, int init) {
     else STEP_LAUNCH(TESS_II, false);
   }
 #undef STEP_LAUNCH
+  reg2d_launch_tables(r, s, D.K);
   if (timed) {
     int rc = prof_end(r->ctx, a, b);
     if (rc) return rc;
@@ -1190,6 +1201,7 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
     while (K_cap > K && (size_t)mcmc_finish_doubles(K_cap, q.d) * sizeof(double) > (size_t)ctx->max_lds) K_cap--;
   }
   if ((size_t)(2 * q.nC * 6) * sizeof(double) > (size_t)ctx->max_lds ||
+      (size_t)mcmc_tables_doubles(q.nC, q.d) * sizeof(double) > (size_t)ctx->max_lds ||
       (size_t)mcmc_finish_doubles(K, q.d) * sizeof(double) > (size_t)ctx->max_lds)
     return fail(ctx, CPAB_ERR_UNSUPPORTED, "per-cell tables or proposal workspace exceed LDS");
   cpab_reg2d* r = new (std::nothrow) cpab_reg2d();
@@ -1254,11 +1266,16 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
     return fail(ctx, CPAB_ERR_ARG, "image too large for the launch grid");
   }
   r->K_cap = K_cap;
-  // scratch, zeroed once: theta_prop | ll_prop | tables | partial | partial_nfb | stop | done
+  // scratch, zeroed once: theta_prop | ll_prop | tables | partial | partial_nfb | Bt | done_slot | stop | done
+  // (in doubles; done_slot starts on a 128-byte boundary)
   const size_t nck = (size_t)q.nChains * K_cap;
-  const size_t n = nck * q.d + q.nChains + nck * 2 * q.nC * 6 + 2 * nck * D.nBlocks + q.nChains + q.nChains;
+  const size_t nB = (size_t)q.nC * 6 * q.d;
+  const size_t o_ds = (nck * q.d + q.nChains + nck * 2 * q.nC * 6 + 2 * nck * D.nBlocks + nB + 15) / 16 * 16;
+  const size_t n_ds = nck * DONE_STRIDE / 2;
+  const size_t n = o_ds + n_ds + q.nChains + q.nChains;
   hipError_t e = hipMalloc(&r->scratch, n * sizeof(double));
-  if (e == hipSuccess) e = hipMemset(r->scratch, 0, n * sizeof(double));
+  // zeroed on the context's stream, so the kernels after it see zeros
+  if (e == hipSuccess) e = hipMemsetAsync(r->scratch, 0, n * sizeof(double), ctx->stream);
   if (e != hipSuccess) {
     if (r->scratch) (void)hipFree(r->scratch);
     delete r;
@@ -1269,9 +1286,21 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   D.tables = D.ll_prop + q.nChains;
   D.partial = D.tables + nck * 2 * q.nC * 6;
   D.partial_nfb = D.partial + nck * D.nBlocks;
-  D.stop = reinterpret_cast<int64_t*>(D.partial_nfb + nck * D.nBlocks);
+  double* Bt = D.partial_nfb + nck * D.nBlocks;
+  D.Bt = Bt;
+  D.done_slot = reinterpret_cast<unsigned int*>(r->scratch + o_ds);
+  D.stop = reinterpret_cast<int64_t*>(r->scratch + o_ds + n_ds);
   D.done = reinterpret_cast<unsigned int*>(D.stop + q.nChains);
   r->host_step.resize(q.nChains);
+  // B is read transposed by the tables kernel (a copy: later changes of B are not seen)
+  hipLaunchKernelGGL(k_transpose, dim3((unsigned)((nB + 255) / 256)), dim3(256), 0, ctx->stream, q.B, Bt,
+                     q.nC * 6, q.d);
+  e = hipGetLastError();
+  if (e != hipSuccess) {
+    (void)hipFree(r->scratch);
+    delete r;
+    return fail(ctx, CPAB_ERR_HIP, "k_transpose: %s", hipGetErrorString(e));
+  }
   *out = r;
   return CPAB_OK;
 }
@@ -1281,6 +1310,7 @@ int cpab_reg2d_init(cpab_reg2d* r) {
   McmcDesc& D = r->D;
   const size_t sh = (size_t)mcmc_arm_doubles(D.K, D.d) * sizeof(double);
   hipLaunchKernelGGL(k_mcmc_prep, dim3(D.nChains), dim3(256), sh, r->ctx->stream, D, 0);
+  reg2d_launch_tables(r, r->ctx->stream, 1);
   HIP_OK(r->ctx, hipGetLastError());
   return reg2d_launch_step(r, r->ctx->stream, 1);
 }
@@ -1352,6 +1382,7 @@ int cpab_reg2d_set_spec_depth(cpab_reg2d* r, int K) {
   // numbers the chain would use, so the chain is unchanged
   const size_t sh = (size_t)mcmc_arm_doubles(D.K, D.d) * sizeof(double);
   hipLaunchKernelGGL(k_mcmc_prep, dim3(D.nChains), dim3(256), sh, ctx->stream, D, 1);
+  reg2d_launch_tables(r, ctx->stream, D.K);
   HIP_OK(ctx, hipGetLastError());
   return CPAB_OK;
 }

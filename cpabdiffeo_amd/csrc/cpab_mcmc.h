@@ -17,7 +17,10 @@
 //             log_ratio (with `inverted` the reference's test log(u) >
 //             log_ratio, Metropolis.py:77-82), slot after slot until the
 //             first rejection, advances the step counter by the slots
-//             decided, and arms the next K proposals for the chain.
+//             decided, and arms the next K proposals (theta') for the chain.
+//   then      k_mcmc_tables, a launch of (slots x chains) blocks, forms the
+//             armed proposals' per-cell tables (B theta', expm) for the next
+//             step launch.
 //
 // Speculation (K > 1, prefetching in the sense of Brockwell, J. Comput.
 // Graph. Stat. 15(1), 2006): slot m holds the proposal of step s + m on the
@@ -26,7 +29,8 @@
 // discarded.  Every decided step uses exactly the numbers the K = 1 chain
 // uses (same proposal sums, same random counters), so the chain is the same
 // bit for bit; only idle GPU capacity is spent on the discarded slots.
-// k_mcmc_prep arms the first (noise-free) proposal before the init launch.
+// k_mcmc_prep (+ k_mcmc_tables) arms the first (noise-free) proposal before
+// the init launch.
 //
 // Random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), keyed by
 // the seed, counter (step, chain, slot); Box-Muller for normals.  Everything
@@ -67,6 +71,9 @@ __host__ __device__ inline double u53(uint32_t a, uint32_t b) {
 }
 
 constexpr uint32_t SLOT_ACCEPT = 0x80000000u;
+// Spacing of the per-slot completion counters (unsigned ints): one 128-byte
+// line per counter.
+constexpr int DONE_STRIDE = 32;
 
 // Threads per block of the registration step: 256 (4 waves) or 64.  A block
 // keeps its CU resources until its slowest wave ends, so small blocks free
@@ -85,6 +92,7 @@ struct McmcDesc {
   uint32_t seed_lo, seed_hi;
   int K;                // proposals evaluated per chain and launch (speculation depth)
   const double* B;      // (nC*6, d) row-major
+  const double* Bt;     // (d, nC*6) row-major: B transposed
   const double* L;      // (d, d) row-major lower triangular
   const float* src;     // (H, W)
   const float* dst;     // (H, W)
@@ -98,7 +106,9 @@ struct McmcDesc {
   double* tables;       // (nChains, K, 2, nC, 6): per slot the Trels then the signed As
   double* partial;      // (nChains, K, nBlocks) block sums of squared errors
   double* partial_nfb;  // (nChains, K, nBlocks) block counts of rejected Trel proposals
-  unsigned int* done;   // (nChains,) blocks finished in the current launch (0 between launches)
+  unsigned int* done;   // (nChains,) slots whose blocks have all finished in the current launch
+  unsigned int* done_slot;  // (nChains, K, DONE_STRIDE) blocks finished per slot in the current
+                            // launch, one 128-byte line each (all 0 between launches)
   double* trace;        // (nChains, trace_len) or null
   unsigned long long* nfb;  // (nChains,) rejected Trel proposals of decided steps, accumulated; or null
   int64_t* stats;       // (nChains, 4) or null: proposals evaluated, their rejected Trel
@@ -108,90 +118,134 @@ struct McmcDesc {
   Geom g;
 };
 
-// LDS of the last block's epilogue (doubles): ll' and fallback count per
-// slot, the decided theta, normals and proposals of every slot.
-__host__ __device__ inline int mcmc_finish_doubles(int K, int d) { return 2 * K + d + 2 * K * d; }
+// LDS of the last block's epilogue (doubles): ll', fallback count and
+// acceptance uniform per slot, the decided theta, normals and proposals of
+// every slot.
+__host__ __device__ inline int mcmc_finish_doubles(int K, int d) { return 3 * K + d + 2 * K * d; }
 // LDS of k_mcmc_prep (doubles): normals and proposals of every slot.
 __host__ __device__ inline int mcmc_arm_doubles(int K, int d) { return 2 * K * d; }
+// LDS of k_mcmc_tables (doubles): one proposal and its Avees.
+__host__ __device__ inline int mcmc_tables_doubles(int nC, int d) { return d + 6 * nC; }
 
 // Arm the K proposals that follow `step` from the state theta, by the whole
 // block.  Slot m is the proposal of step + m on the path where every earlier
 // slot is accepted: theta'_0 = theta + L z_step, theta'_m = theta'_{m-1} +
 // L z_{step+m} -- the same sums, in the same order, as the one-step-at-a-time
 // chain forms them (Proposal.__call__, cpab/cpaNd/inference/Proposal.py:58-64).
-// Then Avees = B theta' (CpaSpace.theta2Avees, _CpaSpace.py:366-376), sign -1,
-// dt scaling and expm per cell (the calc_T_inv prep) into D.tables.
+// The per-cell tables of the armed proposals are k_mcmc_tables' work, a
+// launch of its own that spreads them over many CUs (in the last block of
+// the step they were its serial tail).
 // with_noise == 0 arms slot 0 with theta itself.  z, th: K*d doubles of LDS each.
-__device__ inline void mcmc_arm(const McmcDesc& D, int ch, int64_t step, int with_noise, const double* theta,
-                                double* z, double* th) {
-  const int d = D.d, nC = D.nC, K = D.K;
-  const int nk = with_noise ? K : 1;
+__device__ inline void mcmc_arm_theta(const McmcDesc& D, int ch, int64_t step, int with_noise, const double* theta,
+                                      double* z, double* th) {
+  const int d = D.d, K = D.K;
   double* tp = D.theta_prop + (int64_t)ch * K * d;
-  if (with_noise) {
-    const int hd = (d + 1) / 2;
-    for (int idx = threadIdx.x; idx < K * hd; idx += blockDim.x) {
-      const int m = idx / hd, t = idx - m * hd;
-      const int64_t s = step + m;
-      const Philox r = philox4x32_10((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)ch, (uint32_t)t, D.seed_lo,
-                                     D.seed_hi);
-      const double u1 = 1.0 - u53(r.x0, r.x1), u2 = u53(r.x2, r.x3);
-      const double rad = sqrt(-2.0 * log(u1));
-      const double ang = 6.283185307179586 * u2;
-      z[m * d + 2 * t] = rad * cos(ang);
-      if (2 * t + 1 < d) z[m * d + 2 * t + 1] = rad * sin(ang);
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < K * d; idx += blockDim.x) {
-      const int m = idx / d, i = idx - m * d;
-      double acc = 0.0;
-      for (int j = 0; j <= i; j++) acc += D.L[(int64_t)i * d + j] * z[m * d + j];
-      th[idx] = acc;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < d; i += blockDim.x) {
-      double v = theta[i];
-      for (int m = 0; m < K; m++) {
-        v = v + th[m * d + i];
-        th[m * d + i] = v;
-        tp[m * d + i] = v;
-      }
-    }
-  } else {
-    for (int i = threadIdx.x; i < d; i += blockDim.x) {
-      th[i] = theta[i];
-      tp[i] = theta[i];
-    }
+  if (!with_noise) {
+    for (int i = threadIdx.x; i < d; i += blockDim.x) tp[i] = theta[i];
+    return;
+  }
+  const int hd = (d + 1) / 2;
+  for (int idx = threadIdx.x; idx < K * hd; idx += blockDim.x) {
+    const int m = idx / hd, t = idx - m * hd;
+    const int64_t s = step + m;
+    const Philox r = philox4x32_10((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)ch, (uint32_t)t, D.seed_lo,
+                                   D.seed_hi);
+    const double u1 = 1.0 - u53(r.x0, r.x1), u2 = u53(r.x2, r.x3);
+    const double rad = sqrt(-2.0 * log(u1));
+    const double ang = 6.283185307179586 * u2;
+    z[m * d + 2 * t] = rad * cos(ang);
+    if (2 * t + 1 < d) z[m * d + 2 * t + 1] = rad * sin(ang);
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < nk * nC; idx += blockDim.x) {
-    const int m = idx / nC, c = idx - m * nC;
-    const double* thm = th + m * d;
-    double a[6], adt[6], t6[6];
+  // L z of every slot, summed in index order.  The slot index runs fastest
+  // over the lanes, so a wave's loads of L touch a few rows, and they are
+  // issued eight at a time ahead of the dependent sum.
+  for (int idx = threadIdx.x; idx < K * d; idx += blockDim.x) {
+    const int i = idx / K, m = idx - i * K;
+    const double* Li = D.L + (int64_t)i * d;
+    const double* zm = z + m * d;
+    double acc = 0.0;
+    int j = 0;
+    for (; j + 8 <= i + 1; j += 8) {
+      double l[8];
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-      const double* row = D.B + (int64_t)(c * 6 + k) * d;
-      double acc = 0.0;
-      for (int j = 0; j < d; j++) acc += row[j] * thm[j];
-      double v = acc;
-      v *= -1;  // calc_T_inv
-      a[k] = v;
-      adt[k] = v * D.dt;
+      for (int u = 0; u < 8; u++) l[u] = Li[j + u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) acc += l[u] * zm[j + u];
     }
-    expm_affine2(adt, t6, 12);
-    double* T = D.tables + ((int64_t)ch * K + m) * 2 * nC * 6;
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-      T[c * 6 + k] = t6[k];
-      T[nC * 6 + c * 6 + k] = a[k];
+    for (; j <= i; j++) acc += Li[j] * zm[j];
+    th[m * d + i] = acc;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < d; i += blockDim.x) {
+    double v = theta[i];
+    for (int m = 0; m < K; m++) {
+      v = v + th[m * d + i];
+      tp[m * d + i] = v;
     }
   }
+}
+
+// Per-cell tables of the first nk armed proposals of every chain: grid (nk,
+// chains).  Avees = B theta' (CpaSpace.theta2Avees,
+// cpab/cpaNd/_CpaSpace.py:366-376), each row summed in index order, with B
+// read transposed (Bt, d x 6nC, so the lanes of a wave load consecutive
+// doubles); sign -1, dt scaling and the per-cell expm of the calc_T_inv prep
+// (cpab/cpaNd/_CpaCalcs.py:243-313) into D.tables: Trels, then signed As.
+// The kernel is on the critical path between two step launches and runs on
+// few CUs, so it is written for latency: one row per thread (blocks of up to
+// 1024 threads) and sixteen loads of B in flight per thread.
+__global__ void __launch_bounds__(1024) k_mcmc_tables(McmcDesc D) {
+  extern __shared__ double w[];
+  const int m = blockIdx.x, ch = blockIdx.y;
+  const int d = D.d, nC = D.nC, R = 6 * nC;
+  double* th = w;
+  double* a = w + d;
+  const double* tp = D.theta_prop + ((int64_t)ch * D.K + m) * d;
+  for (int i = threadIdx.x; i < d; i += blockDim.x) th[i] = tp[i];
+  __syncthreads();
+  double* T = D.tables + ((int64_t)ch * D.K + m) * 2 * R;
+  for (int r = threadIdx.x; r < R; r += blockDim.x) {
+    const double* b = D.Bt + r;
+    double acc = 0.0;
+    int j = 0;
+    for (; j + 16 <= d; j += 16) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) v[u] = b[(int64_t)(j + u) * R];
+#pragma unroll
+      for (int u = 0; u < 16; u++) acc += v[u] * th[j + u];
+    }
+    for (; j < d; j++) acc += b[(int64_t)j * R] * th[j];
+    double v = acc;
+    v *= -1;  // calc_T_inv
+    a[r] = v;
+    T[R + r] = v;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < nC; c += blockDim.x) {
+    double adt[6], t6[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) adt[k] = a[c * 6 + k] * D.dt;
+    expm_affine2(adt, t6, 12);
+#pragma unroll
+    for (int k = 0; k < 6; k++) T[c * 6 + k] = t6[k];
+  }
+}
+
+// out (cols x rows) = in (rows x cols) transposed, both row-major.
+__global__ void k_transpose(const double* __restrict__ in, double* __restrict__ out, int rows, int cols) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)rows * cols) return;
+  const int64_t r = i / cols, c = i - r * cols;
+  out[c * rows + r] = in[i];
 }
 
 // One block per chain: arm the proposals of the chain's current step.
 __global__ void __launch_bounds__(256) k_mcmc_prep(McmcDesc D, int with_noise) {
   extern __shared__ double w[];
   const int ch = blockIdx.x;
-  mcmc_arm(D, ch, D.step[ch], with_noise, D.theta + (int64_t)ch * D.d, w, w + D.K * D.d);
+  mcmc_arm_theta(D, ch, D.step[ch], with_noise, D.theta + (int64_t)ch * D.d, w, w + D.K * D.d);
 }
 
 // stop = step + n for every chain (the end of a run of n steps).
@@ -208,7 +262,8 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
   const int K = D.K, d = D.d;
   double* llp = w;
   double* nfs = w + K;
-  double* theta_new = nfs + K;
+  double* lus = nfs + K;
+  double* theta_new = lus + K;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int m = wv; m < kact; m += nw) {
     const double* ps = D.partial + ((int64_t)ch * K + m) * D.nBlocks;
@@ -230,6 +285,15 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
       nfs[m] = nf;
     }
   }
+  // log of every slot's acceptance uniform, in parallel (read only by a slot
+  // whose ll does not improve)
+  if (!init)
+    for (int m = threadIdx.x; m < kact; m += blockDim.x) {
+      const int64_t s = step + m;
+      const Philox r = philox4x32_10((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)ch, SLOT_ACCEPT, D.seed_lo,
+                                     D.seed_hi);
+      lus[m] = log(u53(r.x0, r.x1));
+    }
   __syncthreads();
   if (threadIdx.x == 0) {
     double ll = D.ll[ch];
@@ -246,10 +310,7 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
         if (lr > 0) {
           acc = true;
         } else {
-          const int64_t s = step + m;
-          const Philox r = philox4x32_10((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)ch, SLOT_ACCEPT, D.seed_lo,
-                                         D.seed_hi);
-          const double lu = log(u53(r.x0, r.x1));
+          const double lu = lus[m];
           acc = D.inverted ? (lu > lr) : (lu < lr);
         }
       }
@@ -290,7 +351,7 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
   }
   __syncthreads();
   double* z = theta_new + d;
-  mcmc_arm(D, ch, init ? step : step + nd, 1, theta_new, z, z + K * d);
+  mcmc_arm_theta(D, ch, init ? step : step + nd, 1, theta_new, z, z + K * d);
 }
 
 // One launch evaluates up to K proposals of every chain: grid (K * nBlocks,
@@ -372,9 +433,14 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
       }
     }
   }
-  // block sums (fixed order), published with release; the last block
-  // acquires.  No same-address atomics besides the one completion count per
-  // block: on a multi-XCD part those serialise at the memory side.
+  // block sums (fixed order), published with release; completion is counted
+  // in two levels: every block on its slot's counter (release), the last
+  // block of a slot acquires (a fence) and counts on the chain's counter
+  // (release), and the chain's last block acquires.  Only those K + 1
+  // blocks pay the acquire: at agent scope it invalidates the XCD's L2
+  // (buffer_inv sc1), which per block cost ~0.1 ms per launch at 20 slots x
+  // 256 blocks on MI355X.  Release sequences of the RMW counters and the
+  // fences give the chain's last block every partial (C++ fence rules).
   double nf = (double)nfb;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -397,8 +463,19 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
       __hip_atomic_store(D.partial + at, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(D.partial_nfb + at, sumnf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    const unsigned int prev = __hip_atomic_fetch_add(D.done + ch, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (prev == (unsigned int)(K * D.nBlocks - 1));
+    unsigned int* ds = D.done_slot + ((int64_t)ch * K + slot) * DONE_STRIDE;
+    const unsigned int prev = __hip_atomic_fetch_add(ds, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    int is_last = 0;
+    if (prev == (unsigned int)(D.nBlocks - 1)) {  // the slot's last block
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __hip_atomic_store(ds, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // no other arrival this launch
+      const unsigned int p2 = __hip_atomic_fetch_add(D.done + ch, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (p2 == (unsigned int)(K - 1)) {  // the chain's last block
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        is_last = 1;
+      }
+    }
+    last = is_last;
   }
   __syncthreads();
   if (last) mcmc_finish(D, ch, step, kact, init, lds, fin);  // block-uniform branch

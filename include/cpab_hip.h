@@ -242,7 +242,8 @@ int cpab_calc_T_items(cpab_ctx *ctx, int dim, int tess, int nItems, const cpab_w
  *   accept if ll' > ll, else if log(u) < ll' - ll (log(u) > ll' - ll with
  *   inverted_acceptance, the reference's test, Metropolis.py:77-82).
  * All arrays are device pointers owned by the caller; the object owns only
- * its scratch. -------------------------------------------------------- */
+ * its scratch.  B is copied (transposed) when the object is created, on the
+ * context's stream; later changes to B are not seen. ----------------------- */
 typedef struct {
   int nChains;          /* independent chains sharing the images and basis */
   int d;                /* dimension of theta */
