@@ -1066,7 +1066,7 @@ struct cpab_reg2d {
   int K_cap = 1;  // slots the scratch holds (cpab_reg2d_set_spec_depth may go up to it)
   int block = MCMC_BLOCK;  // threads per block of the step kernel
   bool pow2 = true;
-  double* scratch = nullptr;  // theta_prop | ll_prop | tables | partials | Bt | done_slot | stop | done
+  double* scratch = nullptr;  // theta_prop | ll_prop | tables | partials | slot_res | lz | Bt | done_slot | stop | done
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1076,17 +1076,18 @@ struct cpab_reg2d {
 namespace {
 
 size_t reg2d_lds(const McmcDesc& D) {
-  const int a = 2 * D.nC * 6, b = mcmc_finish_doubles(D.K, D.d);
+  const int a = 2 * D.nC * 6, b = mcmc_finish_doubles(D.K);
   return (size_t)(a > b ? a : b) * sizeof(double);
 }
 
-// Tables of the first nk armed proposals of every chain.
+// Tables of the first nk armed proposals of every chain, and the L z vectors
+// of the chain's current step counter.
 void reg2d_launch_tables(cpab_reg2d* r, hipStream_t s, int nk) {
   const McmcDesc& D = r->D;
   const size_t sh = (size_t)mcmc_tables_doubles(D.nC, D.d) * sizeof(double);
   const int rows = 6 * D.nC;
   const unsigned threads = rows >= 1024 ? 1024u : (unsigned)((rows + 63) / 64 * 64);
-  hipLaunchKernelGGL(k_mcmc_tables, dim3((unsigned)nk, D.nChains), dim3(threads), sh, s, D);
+  hipLaunchKernelGGL(k_mcmc_tables, dim3((unsigned)(nk + D.K), D.nChains), dim3(threads), sh, s, D, nk);
 }
 
 // One step launch and the tables of the proposals it arms.
@@ -1190,19 +1191,19 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
     const int64_t k = kSpecPoints / (nPts * (int64_t)q.nChains);
     K = k < 1 ? 1 : (k > 16 ? 16 : (int)k);
   }
-  // the epilogue's workspace must fit in LDS beside nothing else
-  while (K > 1 && (size_t)mcmc_finish_doubles(K, q.d) * sizeof(double) > (size_t)ctx->max_lds) K--;
+  // the re-arming workspace (normals and proposals of every slot) must fit in LDS
+  while (K > 1 && (size_t)mcmc_arm_doubles(K, q.d) * sizeof(double) > (size_t)ctx->max_lds) K--;
   // room for twice the library's own choice, for later tuning of the depth
   int K_cap = K;
   {
     const int64_t k2 = 2 * kSpecPoints / (nPts * (int64_t)q.nChains);
     const int auto2 = k2 < 1 ? 1 : (k2 > kMaxSpec ? kMaxSpec : (int)k2);
     if (auto2 > K_cap) K_cap = auto2;
-    while (K_cap > K && (size_t)mcmc_finish_doubles(K_cap, q.d) * sizeof(double) > (size_t)ctx->max_lds) K_cap--;
+    while (K_cap > K && (size_t)mcmc_arm_doubles(K_cap, q.d) * sizeof(double) > (size_t)ctx->max_lds) K_cap--;
   }
   if ((size_t)(2 * q.nC * 6) * sizeof(double) > (size_t)ctx->max_lds ||
       (size_t)mcmc_tables_doubles(q.nC, q.d) * sizeof(double) > (size_t)ctx->max_lds ||
-      (size_t)mcmc_finish_doubles(K, q.d) * sizeof(double) > (size_t)ctx->max_lds)
+      (size_t)mcmc_arm_doubles(K, q.d) * sizeof(double) > (size_t)ctx->max_lds)
     return fail(ctx, CPAB_ERR_UNSUPPORTED, "per-cell tables or proposal workspace exceed LDS");
   cpab_reg2d* r = new (std::nothrow) cpab_reg2d();
   if (!r) return CPAB_ERR_NOMEM;
@@ -1266,11 +1267,12 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
     return fail(ctx, CPAB_ERR_ARG, "image too large for the launch grid");
   }
   r->K_cap = K_cap;
-  // scratch, zeroed once: theta_prop | ll_prop | tables | partial | partial_nfb | Bt | done_slot | stop | done
-  // (in doubles; done_slot starts on a 128-byte boundary)
+  // scratch, zeroed once: theta_prop | ll_prop | tables | partial | partial_nfb | slot_res | lz | Bt |
+  // done_slot | stop | done (in doubles; done_slot starts on a 128-byte boundary)
   const size_t nck = (size_t)q.nChains * K_cap;
   const size_t nB = (size_t)q.nC * 6 * q.d;
-  const size_t o_ds = (nck * q.d + q.nChains + nck * 2 * q.nC * 6 + 2 * nck * D.nBlocks + nB + 15) / 16 * 16;
+  const size_t o_ds =
+      (nck * q.d + q.nChains + nck * 2 * q.nC * 6 + 2 * nck * D.nBlocks + 2 * nck + 2 * nck * q.d + nB + 15) / 16 * 16;
   const size_t n_ds = nck * DONE_STRIDE / 2;
   const size_t n = o_ds + n_ds + q.nChains + q.nChains;
   hipError_t e = hipMalloc(&r->scratch, n * sizeof(double));
@@ -1286,7 +1288,9 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   D.tables = D.ll_prop + q.nChains;
   D.partial = D.tables + nck * 2 * q.nC * 6;
   D.partial_nfb = D.partial + nck * D.nBlocks;
-  double* Bt = D.partial_nfb + nck * D.nBlocks;
+  D.slot_res = D.partial_nfb + nck * D.nBlocks;
+  D.lz = D.slot_res + 2 * nck;
+  double* Bt = D.lz + 2 * nck * q.d;
   D.Bt = Bt;
   D.done_slot = reinterpret_cast<unsigned int*>(r->scratch + o_ds);
   D.stop = reinterpret_cast<int64_t*>(r->scratch + o_ds + n_ds);

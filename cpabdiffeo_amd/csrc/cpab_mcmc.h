@@ -10,17 +10,18 @@
 //   body      calc_T_inv of the block's pixels (x = column, y = row); at each
 //             finished point the bilinear sample of the source image and
 //             (warped - target)^2.
-//   epilogue  block sums are published (release), the last block of the chain
-//             (acquire) sums them in a fixed order: ll' = -0.5/sigma^2 sum
-//             (divided by the pixel count when use_ave_ll, Metropolis.py:58-59),
-//             log_ratio = ll' - ll, accept when log_ratio > 0 or log(u) <
-//             log_ratio (with `inverted` the reference's test log(u) >
-//             log_ratio, Metropolis.py:77-82), slot after slot until the
-//             first rejection, advances the step counter by the slots
+//   epilogue  block sums are published (release); the last block of each
+//             slot (acquire) sums them in a fixed order: ll' = -0.5/sigma^2
+//             sum (divided by the pixel count when use_ave_ll,
+//             Metropolis.py:58-59); the last block of the chain (acquire)
+//             forms log_ratio = ll' - ll, accepts when log_ratio > 0 or
+//             log(u) < log_ratio (with `inverted` the reference's test
+//             log(u) > log_ratio, Metropolis.py:77-82), slot after slot until
+//             the first rejection, advances the step counter by the slots
 //             decided, and arms the next K proposals (theta') for the chain.
-//   then      k_mcmc_tables, a launch of (slots x chains) blocks, forms the
+//   then      k_mcmc_tables, a launch of (2K x chains) blocks, forms the
 //             armed proposals' per-cell tables (B theta', expm) for the next
-//             step launch.
+//             step launch and the L z vectors its last block will arm from.
 //
 // Speculation (K > 1, prefetching in the sense of Brockwell, J. Comput.
 // Graph. Stat. 15(1), 2006): slot m holds the proposal of step s + m on the
@@ -105,6 +106,8 @@ struct McmcDesc {
   int64_t* stop;        // (nChains,) step counter at which the current run ends
   double* tables;       // (nChains, K, 2, nC, 6): per slot the Trels then the signed As
   double* partial;      // (nChains, K, nBlocks) block sums of squared errors
+  double* slot_res;     // (nChains, K, 2): ll' and rejected Trel proposals of each evaluated slot
+  double* lz;           // (nChains, 2K, d): L z_{step+j}, j < 2K, for the current step counter
   double* partial_nfb;  // (nChains, K, nBlocks) block counts of rejected Trel proposals
   unsigned int* done;   // (nChains,) slots whose blocks have all finished in the current launch
   unsigned int* done_slot;  // (nChains, K, DONE_STRIDE) blocks finished per slot in the current
@@ -119,22 +122,47 @@ struct McmcDesc {
 };
 
 // LDS of the last block's epilogue (doubles): ll', fallback count and
-// acceptance uniform per slot, the decided theta, normals and proposals of
-// every slot.
-__host__ __device__ inline int mcmc_finish_doubles(int K, int d) { return 3 * K + d + 2 * K * d; }
+// acceptance uniform per slot.
+__host__ __device__ inline int mcmc_finish_doubles(int K) { return 3 * K; }
 // LDS of k_mcmc_prep (doubles): normals and proposals of every slot.
 __host__ __device__ inline int mcmc_arm_doubles(int K, int d) { return 2 * K * d; }
-// LDS of k_mcmc_tables (doubles): one proposal and its Avees.
-__host__ __device__ inline int mcmc_tables_doubles(int nC, int d) { return d + 6 * nC; }
+// LDS of k_mcmc_tables (doubles): one proposal and its Avees, or two normals.
+__host__ __device__ inline int mcmc_tables_doubles(int nC, int d) { return d + 6 * nC > 2 * d ? d + 6 * nC : 2 * d; }
+
+// z_s ~ N(0, I) (d normals, Philox counter (s, chain, pair)) into z.
+__device__ inline void mcmc_normal_pair(const McmcDesc& D, int ch, int64_t s, int t, double* z) {
+  const int d = D.d;
+  const Philox r = philox4x32_10((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)ch, (uint32_t)t, D.seed_lo, D.seed_hi);
+  const double u1 = 1.0 - u53(r.x0, r.x1), u2 = u53(r.x2, r.x3);
+  const double rad = sqrt(-2.0 * log(u1));
+  const double ang = 6.283185307179586 * u2;
+  z[2 * t] = rad * cos(ang);
+  if (2 * t + 1 < d) z[2 * t + 1] = rad * sin(ang);
+}
+
+// (L z)_i, summed in index order, loads of L issued eight at a time ahead of
+// the dependent sum.
+__device__ inline double mcmc_lz_row(const double* __restrict__ Li, const double* zm, int i) {
+  double acc = 0.0;
+  int j = 0;
+  for (; j + 8 <= i + 1; j += 8) {
+    double l[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) l[u] = Li[j + u];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc += l[u] * zm[j + u];
+  }
+  for (; j <= i; j++) acc += Li[j] * zm[j];
+  return acc;
+}
 
 // Arm the K proposals that follow `step` from the state theta, by the whole
 // block.  Slot m is the proposal of step + m on the path where every earlier
 // slot is accepted: theta'_0 = theta + L z_step, theta'_m = theta'_{m-1} +
 // L z_{step+m} -- the same sums, in the same order, as the one-step-at-a-time
 // chain forms them (Proposal.__call__, cpab/cpaNd/inference/Proposal.py:58-64).
-// The per-cell tables of the armed proposals are k_mcmc_tables' work, a
-// launch of its own that spreads them over many CUs (in the last block of
-// the step they were its serial tail).
+// Used where a chain set is (re)armed outside the step launches; the step's
+// last block forms the same sums from L z vectors k_mcmc_tables prepared.
 // with_noise == 0 arms slot 0 with theta itself.  z, th: K*d doubles of LDS each.
 __device__ inline void mcmc_arm_theta(const McmcDesc& D, int ch, int64_t step, int with_noise, const double* theta,
                                       double* z, double* th) {
@@ -147,34 +175,14 @@ __device__ inline void mcmc_arm_theta(const McmcDesc& D, int ch, int64_t step, i
   const int hd = (d + 1) / 2;
   for (int idx = threadIdx.x; idx < K * hd; idx += blockDim.x) {
     const int m = idx / hd, t = idx - m * hd;
-    const int64_t s = step + m;
-    const Philox r = philox4x32_10((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)ch, (uint32_t)t, D.seed_lo,
-                                   D.seed_hi);
-    const double u1 = 1.0 - u53(r.x0, r.x1), u2 = u53(r.x2, r.x3);
-    const double rad = sqrt(-2.0 * log(u1));
-    const double ang = 6.283185307179586 * u2;
-    z[m * d + 2 * t] = rad * cos(ang);
-    if (2 * t + 1 < d) z[m * d + 2 * t + 1] = rad * sin(ang);
+    mcmc_normal_pair(D, ch, step + m, t, z + m * d);
   }
   __syncthreads();
-  // L z of every slot, summed in index order.  The slot index runs fastest
-  // over the lanes, so a wave's loads of L touch a few rows, and they are
-  // issued eight at a time ahead of the dependent sum.
+  // the slot index runs fastest over the lanes, so a wave's loads of L touch
+  // a few rows
   for (int idx = threadIdx.x; idx < K * d; idx += blockDim.x) {
     const int i = idx / K, m = idx - i * K;
-    const double* Li = D.L + (int64_t)i * d;
-    const double* zm = z + m * d;
-    double acc = 0.0;
-    int j = 0;
-    for (; j + 8 <= i + 1; j += 8) {
-      double l[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) l[u] = Li[j + u];
-#pragma unroll
-      for (int u = 0; u < 8; u++) acc += l[u] * zm[j + u];
-    }
-    for (; j <= i; j++) acc += Li[j] * zm[j];
-    th[m * d + i] = acc;
+    th[m * d + i] = mcmc_lz_row(D.L + (int64_t)i * d, z + m * d, i);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < d; i += blockDim.x) {
@@ -186,19 +194,40 @@ __device__ inline void mcmc_arm_theta(const McmcDesc& D, int ch, int64_t step, i
   }
 }
 
-// Per-cell tables of the first nk armed proposals of every chain: grid (nk,
-// chains).  Avees = B theta' (CpaSpace.theta2Avees,
-// cpab/cpaNd/_CpaSpace.py:366-376), each row summed in index order, with B
+// Per-cell tables of the first nk armed proposals of every chain, and the
+// L z vectors the step's last block will need: grid (nk + K, chains).
+// Avees = B theta' (CpaSpace.theta2Avees, cpab/cpaNd/_CpaSpace.py:366-376),
+// each row summed in index order, with B
 // read transposed (Bt, d x 6nC, so the lanes of a wave load consecutive
 // doubles); sign -1, dt scaling and the per-cell expm of the calc_T_inv prep
 // (cpab/cpaNd/_CpaCalcs.py:243-313) into D.tables: Trels, then signed As.
+// Blocks nk .. nk+K-1 form L z_{step+j} for j < 2K (two vectors each) from
+// the step counter the decision left: the last block of the next step arms
+// slot m from vector j0 + m (j0 = steps it decided, 0 after the init launch)
+// -- the sums mcmc_arm_theta forms, in the same order.
 // The kernel is on the critical path between two step launches and runs on
 // few CUs, so it is written for latency: one row per thread (blocks of up to
 // 1024 threads) and sixteen loads of B in flight per thread.
-__global__ void __launch_bounds__(1024) k_mcmc_tables(McmcDesc D) {
+__global__ void __launch_bounds__(1024) k_mcmc_tables(McmcDesc D, int nk) {
   extern __shared__ double w[];
   const int m = blockIdx.x, ch = blockIdx.y;
   const int d = D.d, nC = D.nC, R = 6 * nC;
+  if (m >= nk) {
+    const int q = m - nk;
+    const int64_t step = D.step[ch];
+    const int hd = (d + 1) / 2;
+    for (int idx = threadIdx.x; idx < 2 * hd; idx += blockDim.x) {
+      const int jj = idx / hd, t = idx - jj * hd;
+      mcmc_normal_pair(D, ch, step + 2 * q + jj, t, w + jj * d);
+    }
+    __syncthreads();
+    double* lz = D.lz + ((int64_t)ch * 2 * D.K + 2 * q) * d;
+    for (int idx = threadIdx.x; idx < 2 * d; idx += blockDim.x) {
+      const int i = idx >> 1, jj = idx & 1;
+      lz[jj * d + i] = mcmc_lz_row(D.L + (int64_t)i * d, w + jj * d, i);
+    }
+    return;
+  }
   double* th = w;
   double* a = w + d;
   const double* tp = D.theta_prop + ((int64_t)ch * D.K + m) * d;
@@ -254,46 +283,29 @@ __global__ void k_mcmc_set_stop(McmcDesc D, int64_t n) {
   if (ch < D.nChains) D.stop[ch] = D.step[ch] + n;
 }
 
-// The last block of the chain: reduce every evaluated slot, decide the slots
-// in step order while they stay valid (slot m is valid when slots 0..m-1
-// were accepted), advance, then arm the next K proposals.
+// The last block of the chain: decide the evaluated slots (each reduced by
+// its own last block) in step order while they stay valid (slot m is valid
+// when slots 0..m-1 were accepted), advance, then arm the next K proposals:
+// theta'_m = theta + L z_{s} + ... + L z_{s+m}, s the new step counter, from
+// the L z vectors k_mcmc_tables prepared for this launch.
 __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int kact, int init, double* w,
                                    int* sh) {
   const int K = D.K, d = D.d;
   double* llp = w;
   double* nfs = w + K;
   double* lus = nfs + K;
-  double* theta_new = lus + K;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int m = wv; m < kact; m += nw) {
-    const double* ps = D.partial + ((int64_t)ch * K + m) * D.nBlocks;
-    const double* pn = D.partial_nfb + ((int64_t)ch * K + m) * D.nBlocks;
-    double s = 0.0, nf = 0.0;
-    for (int i = lane; i < D.nBlocks; i += 64) {
-      s += __hip_atomic_load(ps + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      nf += __hip_atomic_load(pn + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      s += __shfl_xor(s, o, 64);
-      nf += __shfl_xor(nf, o, 64);
-    }
-    if (lane == 0) {
-      double l = s * (-0.5 / (D.sigma * D.sigma));
-      if (D.use_ave_ll) l = l / (double)((int64_t)D.H * D.W);
-      llp[m] = l;
-      nfs[m] = nf;
+  for (int m = threadIdx.x; m < kact; m += blockDim.x) {
+    const double* r = D.slot_res + ((int64_t)ch * K + m) * 2;
+    llp[m] = __hip_atomic_load(r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    nfs[m] = __hip_atomic_load(r + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // log of the slot's acceptance uniform (read only if its ll does not improve)
+    if (!init) {
+      const int64_t s = step + m;
+      const Philox u = philox4x32_10((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)ch, SLOT_ACCEPT, D.seed_lo,
+                                     D.seed_hi);
+      lus[m] = log(u53(u.x0, u.x1));
     }
   }
-  // log of every slot's acceptance uniform, in parallel (read only by a slot
-  // whose ll does not improve)
-  if (!init)
-    for (int m = threadIdx.x; m < kact; m += blockDim.x) {
-      const int64_t s = step + m;
-      const Philox r = philox4x32_10((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)ch, SLOT_ACCEPT, D.seed_lo,
-                                     D.seed_hi);
-      lus[m] = log(u53(r.x0, r.x1));
-    }
   __syncthreads();
   if (threadIdx.x == 0) {
     double ll = D.ll[ch];
@@ -340,18 +352,31 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
     }
     D.done[ch] = 0;  // re-arm for the next launch (kernel boundaries order it)
     sh[0] = la;
-    sh[1] = nd;
+    sh[1] = init ? 0 : nd;
   }
   __syncthreads();
-  const int la = sh[0], nd = sh[1];
+  const int la = sh[0], j0 = sh[1];
+  double* tp = D.theta_prop + (int64_t)ch * K * d;
+  const double* lz = D.lz + ((int64_t)ch * 2 * K + j0) * d;
   for (int i = threadIdx.x; i < d; i += blockDim.x) {
-    const double v = la >= 0 ? D.theta_prop[((int64_t)ch * K + la) * d + i] : D.theta[(int64_t)ch * d + i];
-    theta_new[i] = v;
+    double v = la >= 0 ? tp[(int64_t)la * d + i] : D.theta[(int64_t)ch * d + i];
     D.theta[(int64_t)ch * d + i] = v;
+    int m = 0;
+    for (; m + 8 <= K; m += 8) {
+      double t[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) t[u] = lz[(int64_t)(m + u) * d + i];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        v = v + t[u];
+        tp[(int64_t)(m + u) * d + i] = v;
+      }
+    }
+    for (; m < K; m++) {
+      v = v + lz[(int64_t)m * d + i];
+      tp[(int64_t)m * d + i] = v;
+    }
   }
-  __syncthreads();
-  double* z = theta_new + d;
-  mcmc_arm_theta(D, ch, init ? step : step + nd, 1, theta_new, z, z + K * d);
 }
 
 // One launch evaluates up to K proposals of every chain: grid (K * nBlocks,
@@ -365,7 +390,7 @@ template <int TESS, bool POW2, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
   extern __shared__ double lds[];
   __shared__ double red[BLOCK / 64], rednf[BLOCK / 64];
-  __shared__ int last;
+  __shared__ int last, slot_last;
   __shared__ int fin[2];
   const int ch = blockIdx.y;
   const int K = D.K;
@@ -433,14 +458,15 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
       }
     }
   }
-  // block sums (fixed order), published with release; completion is counted
-  // in two levels: every block on its slot's counter (release), the last
-  // block of a slot acquires (a fence) and counts on the chain's counter
-  // (release), and the chain's last block acquires.  Only those K + 1
-  // blocks pay the acquire: at agent scope it invalidates the XCD's L2
-  // (buffer_inv sc1), which per block cost ~0.1 ms per launch at 20 slots x
-  // 256 blocks on MI355X.  Release sequences of the RMW counters and the
-  // fences give the chain's last block every partial (C++ fence rules).
+  // Block sums (fixed order), published with release.  Completion is
+  // counted in two levels: every block on its slot's counter (release); the
+  // slot's last block acquires (a fence), sums the slot's block sums and
+  // counts on the chain's counter (release); the chain's last block
+  // acquires and decides.  Only those K + 1 blocks pay an acquire: at agent
+  // scope it invalidates the XCD's L2 (buffer_inv sc1), which per block cost
+  // ~0.1 ms per launch at 20 slots x 256 blocks on MI355X.  The release
+  // sequences of the counters and the fences order every sum before its
+  // reader (C++ fence rules).
   double nf = (double)nfb;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -452,6 +478,7 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
     rednf[threadIdx.x >> 6] = nf;
   }
   __syncthreads();
+  unsigned int* ds = D.done_slot + ((int64_t)ch * K + slot) * DONE_STRIDE;
   if (threadIdx.x == 0) {
     if (slot < kact) {
       double sum = 0.0, sumnf = 0.0;
@@ -463,19 +490,41 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
       __hip_atomic_store(D.partial + at, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(D.partial_nfb + at, sumnf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    unsigned int* ds = D.done_slot + ((int64_t)ch * K + slot) * DONE_STRIDE;
     const unsigned int prev = __hip_atomic_fetch_add(ds, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    int is_last = 0;
-    if (prev == (unsigned int)(D.nBlocks - 1)) {  // the slot's last block
+    slot_last = prev == (unsigned int)(D.nBlocks - 1);
+    if (slot_last) {  // the slot's last block
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       __hip_atomic_store(ds, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // no other arrival this launch
-      const unsigned int p2 = __hip_atomic_fetch_add(D.done + ch, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      if (p2 == (unsigned int)(K - 1)) {  // the chain's last block
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        is_last = 1;
-      }
     }
-    last = is_last;
+  }
+  __syncthreads();
+  if (!slot_last) return;  // block-uniform
+  if (slot < kact && threadIdx.x < 64) {  // the first wave sums the slot's block sums, lane-strided
+    const int lane = threadIdx.x;
+    const double* ps = D.partial + ((int64_t)ch * K + slot) * D.nBlocks;
+    const double* pn = D.partial_nfb + ((int64_t)ch * K + slot) * D.nBlocks;
+    double s = 0.0, f = 0.0;
+    for (int i = lane; i < D.nBlocks; i += 64) {
+      s += __hip_atomic_load(ps + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      f += __hip_atomic_load(pn + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      s += __shfl_xor(s, o, 64);
+      f += __shfl_xor(f, o, 64);
+    }
+    if (lane == 0) {  // thread 0: its release below publishes these
+      double l = s * (-0.5 / (D.sigma * D.sigma));
+      if (D.use_ave_ll) l = l / (double)((int64_t)D.H * D.W);
+      double* r = D.slot_res + ((int64_t)ch * K + slot) * 2;
+      __hip_atomic_store(r, l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(r + 1, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  if (threadIdx.x == 0) {
+    const unsigned int p2 = __hip_atomic_fetch_add(D.done + ch, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    last = p2 == (unsigned int)(K - 1);
+    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the chain's last block
   }
   __syncthreads();
   if (last) mcmc_finish(D, ch, step, kact, init, lds, fin);  // block-uniform branch
