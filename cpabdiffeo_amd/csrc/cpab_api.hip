@@ -1243,6 +1243,7 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   D.stats = q.stats;
   D.K = K;
   D.g = g;
+  D.tile8 = (q.W % 8 == 0 && q.H % 8 == 0) ? 1 : 0;
   // One pixel per lane while a launch holds few waves per SIMD (latency
   // hiding needs the waves); with many (batched chains) 2 or 4 pixels per
   // lane with refill keep lanes busy past their slowest pixel (measured on

@@ -118,6 +118,7 @@ struct McmcDesc {
                         // proposals, launches, steps decided
   int nBlocks;          // integration blocks per chain and slot
   int ptsPerWave;
+  int tile8;            // 1: pixels are taken in 8 x 8 tiles (W and H multiples of 8)
   Geom g;
 };
 
@@ -382,7 +383,10 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
 // One launch evaluates up to K proposals of every chain: grid (K * nBlocks,
 // nChains), block b integrates pixel range b / K of slot b % K, so the
 // blocks of one pixel range (the slowest one holds the domain corner) are
-// dispatched together.  init == 1: evaluate the armed (noise-free) slot 0 as
+// dispatched together.  A wave's 64 pixels are an 8 x 8 tile when the image
+// allows: neighbouring pixels flow alike and cross facets at about the same
+// steps, so the lanes of a wave need similar numbers of trips (one chain,
+// 20 slots: +11 % over 64 pixels of a row, measured on MI355X).  init == 1: evaluate the armed (noise-free) slot 0 as
 // the current state.  A chain whose run has ended (step >= stop) is left as
 // it is.  Every block of a live chain, evaluating or not, counts its arrival,
 // so the last arrival is the only block that touches the chain's state.
@@ -424,12 +428,12 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
       double p[2];
       int c = 0, t = 0, k = 0;
       const double* row = Ttab;
-      int64_t my = 0;
+      int64_t my = 0, pix = 0;  // work index, pixel index
       bool active = false;
       for (;;) {
         if (active && t >= nT) {
           const float w = sample2(D.src, D.H, D.W, p[0], p[1]);
-          const double e = (double)w - (double)D.dst[my];
+          const double e = (double)w - (double)D.dst[pix];
           acc += e * e;
           active = false;
         }
@@ -440,8 +444,15 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
             if (cand < end) {
               my = cand;
               active = true;
-              p[0] = (double)(my % D.W);
-              p[1] = (double)(my / D.W);
+              if (D.tile8) {
+                const int tt = (int)(my >> 6), u = (int)(my & 63), tw = D.W >> 3;
+                const int px = (tt % tw) * 8 + (u & 7), py = (tt / tw) * 8 + (u >> 3);
+                pix = (int64_t)py * D.W + px;
+              } else {
+                pix = my;
+              }
+              p[0] = (double)(pix % D.W);
+              p[1] = (double)(pix / D.W);
               c = cell2<TESS, POW2>(p, g);
               row = Ttab + mul24(c, 6);
               t = 0;

@@ -278,10 +278,12 @@ typedef struct {
   int pts_per_wave;     /* pixels a wave integrates (a multiple of 64, at most 4096);
                            0 = chosen by the library from the launch size (64, 128 or
                            256 pixels per wave; many chains one step per launch also
-                           get one-wave blocks).  The log-likelihood's summation order
-                           follows the pixels' assignment to blocks, so chains give
-                           the same bits alone and in a batch when they run with the
-                           same explicit value (4-wave blocks then). */
+                           get one-wave blocks).  Pixels go to waves in 8 x 8 tiles
+                           when W and H are multiples of 8, else in row order.  The
+                           log-likelihood's summation order follows the pixels'
+                           assignment to blocks, so chains give the same bits alone
+                           and in a batch when they run with the same explicit value
+                           (4-wave blocks then). */
 } cpab_reg2d_desc;
 
 typedef struct cpab_reg2d cpab_reg2d;

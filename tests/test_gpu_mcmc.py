@@ -33,11 +33,19 @@ def test_target_image_matches_oracle(problem, orc):
     assert abs(noise.std() - 0.02) < 0.002  # the rest is the N(0, 0.02^2) noise
 
 
-@pytest.mark.parametrize("spec", [1, 4])
-def test_chain_matches_cpu_reference(problem, orc, gpu, spec):
+@pytest.fixture(scope="module")
+def problem60(gpu):
+    """60 x 60 pixels: not a multiple of 8, so pixels go to waves in row order
+    (and the cells are 15 pixels wide: non-power-of-two lookups)."""
+    from cpabdiffeo_amd.registration import RegistrationProblem
+    return RegistrationProblem(n=60, seed=4, nTimeSteps=50, nStepsODE=10)
+
+
+@pytest.mark.parametrize("spec,size", [(1, 64), (4, 64), (1, 60), (4, 60)])
+def test_chain_matches_cpu_reference(problem, problem60, orc, gpu, spec, size):
     from cpabdiffeo_amd.registration import DeviceMetropolis
-    say("chain vs CPU reference, 2 chains x 12 steps, spec_depth", spec)
-    p = problem
+    say("chain vs CPU reference, 2 chains x 12 steps, spec_depth", spec, "image", size)
+    p = problem if size == 64 else problem60
     nsteps = 12
     mh = DeviceMetropolis(p, nChains=2, scale=0.05, seed=11, trace_len=nsteps, spec_depth=spec)
     theta0 = np.zeros(mh.d)
