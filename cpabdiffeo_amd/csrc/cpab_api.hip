@@ -1080,14 +1080,15 @@ size_t reg2d_lds(const McmcDesc& D) {
   return (size_t)(a > b ? a : b) * sizeof(double);
 }
 
-// Tables of the first nk armed proposals of every chain, and the L z vectors
-// of the chain's current step counter.
+// Tables of the first nk armed proposals of every chain, the L z vectors of
+// the chain's current step counter and the next dispatch order.
 void reg2d_launch_tables(cpab_reg2d* r, hipStream_t s, int nk) {
   const McmcDesc& D = r->D;
-  const size_t sh = (size_t)mcmc_tables_doubles(D.nC, D.d) * sizeof(double);
+  const size_t sh = (size_t)mcmc_tables_doubles(D.nC, D.d, D.nBlocks) * sizeof(double);
   const int rows = 6 * D.nC;
   const unsigned threads = rows >= 1024 ? 1024u : (unsigned)((rows + 63) / 64 * 64);
-  hipLaunchKernelGGL(k_mcmc_tables, dim3((unsigned)(nk + D.K), D.nChains), dim3(threads), sh, s, D, nk);
+  hipLaunchKernelGGL(k_mcmc_tables, dim3((unsigned)(nk + D.K + (D.order ? 1 : 0)), D.nChains), dim3(threads), sh, s,
+                     D, nk);
 }
 
 // One step launch and the tables of the proposals it arms.
@@ -1202,7 +1203,7 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
     while (K_cap > K && (size_t)mcmc_arm_doubles(K_cap, q.d) * sizeof(double) > (size_t)ctx->max_lds) K_cap--;
   }
   if ((size_t)(2 * q.nC * 6) * sizeof(double) > (size_t)ctx->max_lds ||
-      (size_t)mcmc_tables_doubles(q.nC, q.d) * sizeof(double) > (size_t)ctx->max_lds ||
+      (size_t)mcmc_tables_doubles(q.nC, q.d, 1024) * sizeof(double) > (size_t)ctx->max_lds ||
       (size_t)mcmc_arm_doubles(K, q.d) * sizeof(double) > (size_t)ctx->max_lds)
     return fail(ctx, CPAB_ERR_UNSUPPORTED, "per-cell tables or proposal workspace exceed LDS");
   cpab_reg2d* r = new (std::nothrow) cpab_reg2d();
@@ -1275,7 +1276,8 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   const size_t o_ds =
       (nck * q.d + q.nChains + nck * 2 * q.nC * 6 + 2 * nck * D.nBlocks + 2 * nck + 2 * nck * q.d + nB + 15) / 16 * 16;
   const size_t n_ds = nck * DONE_STRIDE / 2;
-  const size_t n = o_ds + n_ds + q.nChains + q.nChains;
+  const size_t n_ord = ((size_t)q.nChains * D.nBlocks + 1) / 2;  // ints, in doubles
+  const size_t n = o_ds + n_ds + q.nChains + q.nChains + n_ord;
   hipError_t e = hipMalloc(&r->scratch, n * sizeof(double));
   // zeroed on the context's stream, so the kernels after it see zeros
   if (e == hipSuccess) e = hipMemsetAsync(r->scratch, 0, n * sizeof(double), ctx->stream);
@@ -1296,6 +1298,8 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   D.done_slot = reinterpret_cast<unsigned int*>(r->scratch + o_ds);
   D.stop = reinterpret_cast<int64_t*>(r->scratch + o_ds + n_ds);
   D.done = reinterpret_cast<unsigned int*>(D.stop + q.nChains);
+  // blocks in dispatch order by their work when lanes take several pixels
+  D.order = D.ptsPerWave > 64 ? reinterpret_cast<int*>(r->scratch + o_ds + n_ds + 2 * q.nChains) : nullptr;
   r->host_step.resize(q.nChains);
   // B is read transposed by the tables kernel (a copy: later changes of B are not seen)
   hipLaunchKernelGGL(k_transpose, dim3((unsigned)((nB + 255) / 256)), dim3(256), 0, ctx->stream, q.B, Bt,

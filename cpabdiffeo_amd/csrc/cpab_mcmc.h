@@ -119,6 +119,8 @@ struct McmcDesc {
   int nBlocks;          // integration blocks per chain and slot
   int ptsPerWave;
   int tile8;            // 1: pixels are taken in 8 x 8 tiles (W and H multiples of 8)
+  int* order;           // (nChains, nBlocks): the pixel block each dispatch position integrates,
+                        // or null: dispatch position = pixel block
   Geom g;
 };
 
@@ -127,8 +129,13 @@ struct McmcDesc {
 __host__ __device__ inline int mcmc_finish_doubles(int K) { return 3 * K; }
 // LDS of k_mcmc_prep (doubles): normals and proposals of every slot.
 __host__ __device__ inline int mcmc_arm_doubles(int K, int d) { return 2 * K * d; }
-// LDS of k_mcmc_tables (doubles): one proposal and its Avees, or two normals.
-__host__ __device__ inline int mcmc_tables_doubles(int nC, int d) { return d + 6 * nC > 2 * d ? d + 6 * nC : 2 * d; }
+// LDS of k_mcmc_tables (doubles): one proposal and its Avees, two normals, or
+// the block counts of the dispatch ordering.
+__host__ __device__ inline int mcmc_tables_doubles(int nC, int d, int nBlocks) {
+  int n = d + 6 * nC > 2 * d ? d + 6 * nC : 2 * d;
+  const int o = nBlocks < 1024 ? nBlocks : 1024;
+  return n > o ? n : o;
+}
 
 // z_s ~ N(0, I) (d normals, Philox counter (s, chain, pair)) into z.
 __device__ inline void mcmc_normal_pair(const McmcDesc& D, int ch, int64_t s, int t, double* z) {
@@ -195,8 +202,41 @@ __device__ inline void mcmc_arm_theta(const McmcDesc& D, int ch, int64_t step, i
   }
 }
 
-// Per-cell tables of the first nk armed proposals of every chain, and the
-// L z vectors the step's last block will need: grid (nk + K, chains).
+// Dispatch order of a chain's pixel blocks for the next step launch, when
+// lanes take several pixels each (refill keeps them busy, so a block's
+// duration follows its total work): by descending count of rejected Trel
+// proposals in slot 0 of the last launch (ties by index).  The longest
+// blocks start first and the shortest fill the end of the launch
+// (longest-processing-time first; 256 chains one step per launch: +7 %,
+// measured on MI355X).  With one pixel per lane a block's duration follows
+// its slowest pixel, and ordering by that measured no faster than the
+// identity (the corner pixel's block, the longest chain of trips, is
+// dispatched first anyway), so such launches keep the identity and skip the
+// lookup.  Only the timing depends on the order: every block stores its
+// sums at its pixel block's index.  Above kOrderMax blocks the quadratic
+// ranking is skipped and the order is the identity.
+constexpr int kOrderMax = 1024;
+__device__ inline void mcmc_order(const McmcDesc& D, int ch, double* w) {
+  const int n = D.nBlocks;
+  int* ord = D.order + (int64_t)ch * n;
+  if (n > kOrderMax) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) ord[i] = i;
+    return;
+  }
+  const double* nf = D.partial_nfb + (int64_t)ch * D.K * n;  // slot 0
+  for (int i = threadIdx.x; i < n; i += blockDim.x) w[i] = nf[i];
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = w[i];
+    int rank = 0;
+    for (int j = 0; j < n; j++) rank += (w[j] > v) | ((w[j] == v) & (j < i));
+    ord[rank] = i;
+  }
+}
+
+// Per-cell tables of the first nk armed proposals of every chain, the L z
+// vectors the step's last block will need, and the next dispatch order:
+// grid (nk + K (+ 1 with an order), chains).
 // Avees = B theta' (CpaSpace.theta2Avees, cpab/cpaNd/_CpaSpace.py:366-376),
 // each row summed in index order, with B
 // read transposed (Bt, d x 6nC, so the lanes of a wave load consecutive
@@ -213,6 +253,10 @@ __global__ void __launch_bounds__(1024) k_mcmc_tables(McmcDesc D, int nk) {
   extern __shared__ double w[];
   const int m = blockIdx.x, ch = blockIdx.y;
   const int d = D.d, nC = D.nC, R = 6 * nC;
+  if (m == nk + D.K) {  // dispatch order of the next step's pixel blocks (when D.order)
+    mcmc_order(D, ch, w);
+    return;
+  }
   if (m >= nk) {
     const int q = m - nk;
     const int64_t step = D.step[ch];
@@ -386,9 +430,9 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
 // dispatched together.  A wave's 64 pixels are an 8 x 8 tile when the image
 // allows: neighbouring pixels flow alike and cross facets at about the same
 // steps, so the lanes of a wave need similar numbers of trips (one chain,
-// 20 slots: +11 % over 64 pixels of a row, measured on MI355X).  init == 1: evaluate the armed (noise-free) slot 0 as
-// the current state.  A chain whose run has ended (step >= stop) is left as
-// it is.  Every block of a live chain, evaluating or not, counts its arrival,
+// 20 slots: +11 % over 64 pixels of a row, measured on MI355X).  init == 1:
+// evaluate the armed (noise-free) slot 0 as the current state.  A chain
+// whose run has ended (step >= stop) is left as it is.  Every block of a live chain, evaluating or not, counts its arrival,
 // so the last arrival is the only block that touches the chain's state.
 template <int TESS, bool POW2, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
@@ -399,7 +443,8 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
   const int ch = blockIdx.y;
   const int K = D.K;
   const int slot = (int)(blockIdx.x % (unsigned)K);
-  const int bx = (int)(blockIdx.x / (unsigned)K);
+  const int bx = D.order ? D.order[(int64_t)ch * D.nBlocks + (int)(blockIdx.x / (unsigned)K)]
+                         : (int)(blockIdx.x / (unsigned)K);
   const int nC = D.nC;
   const int64_t step = D.step[ch];
   int kact = 1;

@@ -41,13 +41,14 @@ def problem60(gpu):
     return RegistrationProblem(n=60, seed=4, nTimeSteps=50, nStepsODE=10)
 
 
-@pytest.mark.parametrize("spec,size", [(1, 64), (4, 64), (1, 60), (4, 60)])
-def test_chain_matches_cpu_reference(problem, problem60, orc, gpu, spec, size):
+@pytest.mark.parametrize("spec,size,ppw", [(1, 64, 0), (4, 64, 0), (1, 60, 0), (4, 60, 0), (4, 64, 128)])
+def test_chain_matches_cpu_reference(problem, problem60, orc, gpu, spec, size, ppw):
+    """ppw 128: two pixels per lane, pixel blocks dispatched in work order."""
     from cpabdiffeo_amd.registration import DeviceMetropolis
-    say("chain vs CPU reference, 2 chains x 12 steps, spec_depth", spec, "image", size)
+    say("chain vs CPU reference, 2 chains x 12 steps, spec_depth", spec, "image", size, "pixels per wave", ppw)
     p = problem if size == 64 else problem60
     nsteps = 12
-    mh = DeviceMetropolis(p, nChains=2, scale=0.05, seed=11, trace_len=nsteps, spec_depth=spec)
+    mh = DeviceMetropolis(p, nChains=2, scale=0.05, seed=11, trace_len=nsteps, spec_depth=spec, pts_per_wave=ppw)
     theta0 = np.zeros(mh.d)
     mh.set_theta(theta0)
     ll0 = log_likelihood(orc, p.space, p.B, theta0, p.src, p.dst, p.sigma, p.params)
