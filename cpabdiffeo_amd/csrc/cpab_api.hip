@@ -138,6 +138,7 @@ struct Shape {
   int64_t grid;
   size_t lds;
   bool use_lds;
+  bool fast;  // enough waves per SIMD to be throughput-bound: the FAST 2D lookup pays
 };
 
 Shape integrate_shape(const cpab_ctx* ctx, int64_t nPts, int nC, int ne, int ntables) {
@@ -149,12 +150,12 @@ Shape integrate_shape(const cpab_ctx* ctx, int64_t nPts, int nC, int ne, int nta
   // waves wanted: 8 per SIMD (4 SIMDs per CU); each lane takes ceil of the rest
   const int64_t want_waves = (int64_t)ctx->n_cu * 4 * 8;
   int64_t per_lane = (nPts + want_waves * 64 - 1) / (want_waves * 64);
-  if (const char* e = getenv("CPAB_PTS_PER_LANE")) per_lane = atoi(e);
   if (per_lane < 1) per_lane = 1;
   s.ptsPerWave = (int)(per_lane * 64);
   const int64_t waves = (nPts + s.ptsPerWave - 1) / s.ptsPerWave;
   const int wpb = s.block / 64;
   s.grid = (waves + wpb - 1) / wpb;
+  s.fast = waves >= (int64_t)ctx->n_cu * 4 * 4;  // four waves per SIMD or more
   return s;
 }
 
@@ -190,6 +191,16 @@ template <int DIM, int TESS, bool POW2, bool LDS>
 int launch_integrate_t(cpab_ctx* ctx, int mode, const Shape& s, const double* pos0, double* out,
                        const double* Trels, const double* As, double h, int nT, int nS, int64_t nPts,
                        const Geom& g, int nC, int32_t* nfb) {
+  if constexpr (DIM == 2 && TESS == TESS_I && POW2) {
+    if (mode == 0 && s.fast) {
+      auto k = k_integrate<DIM, TESS, POW2, LDS, 0, true>;
+      allow_lds(k, s.lds);
+      hipLaunchKernelGGL(k, dim3((unsigned)s.grid), dim3(s.block), s.lds, ctx->stream, pos0, out, Trels, As, h, nT,
+                         nS, nPts, g, nC, s.ptsPerWave, nfb);
+      HIP_OK(ctx, hipGetLastError());
+      return CPAB_OK;
+    }
+  }
   if (mode == 0) {
     auto k = k_integrate<DIM, TESS, POW2, LDS, 0>;
     allow_lds(k, s.lds);
@@ -1105,8 +1116,8 @@ int reg2d_launch_step(cpab_reg2d* r, hipStream_t s, int init) {
   }
 #define STEP_LAUNCH(T_, P_)                                                                            \
   do {                                                                                                 \
-    if (r->block == 64) hipLaunchKernelGGL((k_mcmc_step<T_, P_, 64>), grid, dim3(64), sh, s, D, init); \
-    else hipLaunchKernelGGL((k_mcmc_step<T_, P_, 256>), grid, dim3(256), sh, s, D, init);           \
+    if (r->block == 64) hipLaunchKernelGGL((k_mcmc_step<T_, P_, 64, true>), grid, dim3(64), sh, s, D, init); \
+    else hipLaunchKernelGGL((k_mcmc_step<T_, P_, 256, false>), grid, dim3(256), sh, s, D, init);          \
   } while (0)
   if (D.tess == TESS_I) {
     if (r->pow2) STEP_LAUNCH(TESS_I, true);

@@ -242,7 +242,7 @@ __device__ __forceinline__ void stage_tables(double* lds, const double* __restri
 // and selecting, or chaining several Trel proposals speculatively so their
 // lookups overlap, are both slower than this.)
 // Returns true when the step is complete (the caller advances t).
-template <int DIM, int TESS, bool POW2>
+template <int DIM, int TESS, bool POW2, bool FAST = false>
 __device__ __forceinline__ bool trip(double* p, int& c, int& k, const double*& row, int& nfb, const double* Ttab,
                                      const double* Atab, double h, int nS, const Geom& g) {
   constexpr int NE = DIM * (DIM + 1);
@@ -257,7 +257,7 @@ __device__ __forceinline__ bool trip(double* p, int& c, int& k, const double*& r
 #pragma unroll
     for (int d = 0; d < DIM; d++) y[d] = p[d] + y[d] * h;
   }
-  const int cy = cell_of<DIM, TESS, POW2>(y, g);
+  const int cy = cell_of<DIM, TESS, POW2, FAST>(y, g);
   if (!fast) {
 #pragma unroll
     for (int d = 0; d < DIM; d++) p[d] = y[d];
@@ -295,7 +295,7 @@ __device__ __forceinline__ bool trip(double* p, int& c, int& k, const double*& r
 // ballots and branches, so it stays out of the per-trip path).
 // One wave integrates the points [begin, end) of pos0 (indices into pos0,
 // out and nfb_out).  Wave-uniform arguments; every lane of the wave calls it.
-template <int DIM, int TESS, bool POW2, int MODE>
+template <int DIM, int TESS, bool POW2, int MODE, bool FAST = false>
 __device__ __forceinline__ void integrate_range(int64_t begin, int64_t end, const double* __restrict__ pos0,
                                                 double* __restrict__ out, const double* Ttab, const double* Atab,
                                                 double h, int nT, int nS, int64_t nPts, const Geom& g,
@@ -334,7 +334,7 @@ __device__ __forceinline__ void integrate_range(int64_t begin, int64_t end, cons
     if (!active || t >= nT) continue;
 #pragma unroll 2
     do {  // trips until some lane of the region finishes its point
-      if (trip<DIM, TESS, POW2>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) {
+      if (trip<DIM, TESS, POW2, FAST>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) {
         if (MODE == 1) store_pt<DIM>(out, (int64_t)t * nPts + my, p);
         t++;
       }
@@ -342,7 +342,7 @@ __device__ __forceinline__ void integrate_range(int64_t begin, int64_t end, cons
   }
 }
 
-template <int DIM, int TESS, bool POW2, bool LDS, int MODE>
+template <int DIM, int TESS, bool POW2, bool LDS, int MODE, bool FAST = false>
 __global__ void __launch_bounds__(1024)
     k_integrate(const double* __restrict__ pos0, double* __restrict__ out,
                 const double* __restrict__ Trels, const double* __restrict__ As, double h,
@@ -361,7 +361,7 @@ __global__ void __launch_bounds__(1024)
   const int64_t begin = wave * (int64_t)ptsPerWave;
   if (begin >= nPts) return;
   const int64_t end = begin + ptsPerWave < nPts ? begin + ptsPerWave : nPts;
-  integrate_range<DIM, TESS, POW2, MODE>(begin, end, pos0, out, Ttab, Atab, h, nT, nS, nPts, g, nfb_out);
+  integrate_range<DIM, TESS, POW2, MODE, FAST>(begin, end, pos0, out, Ttab, Atab, h, nT, nS, nPts, g, nfb_out);
 }
 
 // ---------------------------------------------------------- sensitivity

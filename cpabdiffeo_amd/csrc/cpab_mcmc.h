@@ -434,7 +434,7 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
 // evaluate the armed (noise-free) slot 0 as the current state.  A chain
 // whose run has ended (step >= stop) is left as it is.  Every block of a live chain, evaluating or not, counts its arrival,
 // so the last arrival is the only block that touches the chain's state.
-template <int TESS, bool POW2, int BLOCK>
+template <int TESS, bool POW2, int BLOCK, bool FAST>
 __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
   extern __shared__ double lds[];
   __shared__ double red[BLOCK / 64], rednf[BLOCK / 64];
@@ -498,7 +498,7 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
               }
               p[0] = (double)(pix % D.W);
               p[1] = (double)(pix / D.W);
-              c = cell2<TESS, POW2>(p, g);
+              c = cell2<TESS, POW2, FAST>(p, g);
               row = Ttab + mul24(c, 6);
               t = 0;
               k = 0;
@@ -509,7 +509,7 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
         if (ballot(active) == 0) break;
         if (!active || t >= nT) continue;
         do {
-          if (trip<2, TESS, POW2>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) t++;
+          if (trip<2, TESS, POW2, FAST>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) t++;
         } while (ballot(t >= nT) == 0);
       }
     }

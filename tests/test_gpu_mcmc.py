@@ -170,3 +170,29 @@ def test_chain_alone_equals_chain_in_a_batch_at_equal_pixels_per_wave(problem, g
         res.append((mh.theta.cpu().numpy()[0].copy(), mh.trace.cpu().numpy()[0].copy()))
         mh.close()
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+
+
+def test_many_chains_one_step_per_launch_match_cpu_reference(orc, gpu):
+    """256 chains of 128 x 128 pixels, one step per launch: the library's
+    many-chain configuration (one-wave blocks, four pixels per lane, blocks
+    in work order, the wave-uniform fast lookup).  Two of the chains against
+    the CPU reference chain."""
+    from cpabdiffeo_amd.registration import DeviceMetropolis, RegistrationProblem
+    p = RegistrationProblem(n=128, seed=6, nTimeSteps=50, nStepsODE=10)
+    nsteps = 6
+    say("256 chains x 128x128 x", nsteps, "steps vs CPU reference")
+    mh = DeviceMetropolis(p, nChains=256, scale=0.05, seed=13, trace_len=nsteps, spec_depth=1)
+    theta0 = np.zeros(mh.d)
+    mh.set_theta(theta0)
+    mh.run(nsteps)
+    trace = mh.trace.cpu().numpy()
+    theta = mh.theta.cpu().numpy()
+    nacc = mh.n_accepted.cpu().numpy()
+    mh.close()
+    for ch in (0, 255):
+        ref = run_chain(orc, p.space, p.B, mh.Lprop, theta0, p.src, p.dst, p.sigma, p.params, nsteps, seed=13,
+                        chain=ch)
+        assert nacc[ch] == sum(r[0] for r in ref), ch
+        np.testing.assert_allclose(trace[ch], [r[2] for r in ref], rtol=1e-9)
+        np.testing.assert_allclose(theta[ch], ref[-1][1], rtol=1e-12, atol=1e-14)
+    assert 0 < nacc.sum() < 256 * nsteps
