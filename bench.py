@@ -46,18 +46,19 @@ F_REMAP_SSD = 16         # bilinear sample (13) + squared error accumulation (3)
 # midpoint 6, g2 2, L^T g2 6, g1 4, lambda update 10, dJ/dA accumulation 20
 F_ADJ_SUB = 71
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
-# HBM bytes of the step kernel per evaluated proposal, from the FETCH_SIZE /
-# WRITE_SIZE counter runs (16 proposals per launch) scaled by a plain copy
-# (committed summary); a launch's traffic is that times the proposals it evaluates.
+# HBM bytes of one step launch at each speculation depth, from the FETCH_SIZE /
+# WRITE_SIZE counter runs scaled by a plain copy (committed summary).
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_mcmc_step_registration256_spec.md")
 
 
-def measured_traffic(proposals_per_launch):
+def measured_traffic(depth):
+    """Measured bytes per launch at this depth, or None when not measured."""
     import re
     if not os.path.exists(TRAFFIC_SUMMARY):
         return None
-    m = re.search(r"traffic_bytes_per_proposal = (\d+)", open(TRAFFIC_SUMMARY).read())
-    return int(m.group(1)) * proposals_per_launch if m else None
+    table = {int(k): int(v) for k, v in
+             re.findall(r"traffic_bytes_per_launch\[(\d+)\] = (\d+)", open(TRAFFIC_SUMMARY).read())}
+    return table.get(int(depth))
 
 
 def log(*a):
@@ -443,8 +444,9 @@ def main():
             "proposals_per_sec": args.steps / seq["elapsed"], "speedup_of_speculation": seq["elapsed"] / elapsed,
             "same_chain_bit_for_bit": same_chain},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(st[0] / launches),
-                     "kernel": "k_mcmc_step (K proposals: calc_T_inv + remap + SSD; accept; arm next)",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(spec["K"]),
+                     "kernel": "k_mcmc_step + k_mcmc_tables (K proposals: calc_T_inv + remap + SSD; accept; "
+                               "arm next)",
                      "kernel_us": kern_s * 1e6, "launches": launches,
                      "proposals_evaluated_per_launch": st[0] / launches,
                      "steps_decided_per_launch": st[3] / launches,

@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(1024)
       const int64_t b = it.begin + (int64_t)w * per;
       const int64_t e = b + per < it.end ? b + per : it.end;
       if (b < e)
-        integrate_range<DIM, TESS, POW2, 0, true>(b, e, it.pos0, it.out, lds, lds + nC * NE, h, nT, nS, 0, g, nullptr);
+        integrate_range<DIM, TESS, POW2, 0, false>(b, e, it.pos0, it.out, lds, lds + nC * NE, h, nT, nS, 0, g, nullptr);
     }
   }
 }
