@@ -78,9 +78,8 @@ constexpr int DONE_STRIDE = 32;
 
 // Threads per block of the registration step: 256 (4 waves) or 64.  A block
 // keeps its CU resources until its slowest wave ends, so small blocks free
-// them sooner, but the last block of a chain arms the next proposals with
-// its threads.  Measured on MI355X: one chain with speculation is fastest at
-// 256 (64: -50 %, 1 024: -20 %), 256 chains one step per launch at 64 (+12 %).
+// them sooner.  Measured on MI355X: one chain with speculation is fastest at
+// 256 (64: -2 %, 1 024: -20 %), 256 chains one step per launch at 64 (+12 %).
 constexpr int MCMC_BLOCK = 256;
 
 struct McmcDesc {
