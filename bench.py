@@ -369,7 +369,7 @@ def main():
         return out
 
     log("bench: warmup %d steps, then the timed %d-step chain (speculative prefetch)" % (args.warmup, args.steps))
-    spec = chain(0, args.warmup, args.steps, True, tune=(8, 12, 16, 20, 24, 32))
+    spec = chain(0, args.warmup, args.steps, True, tune=(12, 16, 24, 32, 40, 48))
     seq = None
     if not args.no_seq:
         log("bench: the same chain one step per launch")
@@ -384,11 +384,14 @@ def main():
     # on the launch stream around every launch / graph replay, and the
     # kernel's own counts of evaluated proposals and their fallback steps.
     cnt, ms = spec["events"]
-    st = spec["stats"]  # proposals evaluated, their fallbacks, launches, steps decided
+    # stats: proposals evaluated in full, rejected Trel proposals of every pixel
+    # integrated, launches, steps decided, pixels integrated (slots above a
+    # known rejection are not started, so the work done is what was counted)
+    st = spec["stats"]
     launches = int(st[2])
     kern_s = ms / cnt * 1e-3
     npts = n * n
-    flops_per_launch = (st[0] * npts * (100 * F_FAST + F_REMAP_SSD) + st[1] * 10 * F_SUB) / launches
+    flops_per_launch = (st[4] * (100 * F_FAST + F_REMAP_SSD) + st[1] * 10 * F_SUB) / launches
     achieved = flops_per_launch / kern_s / 1e12
 
     extra = gd = batching = serve = None
@@ -449,6 +452,7 @@ def main():
                                "arm next)",
                      "kernel_us": kern_s * 1e6, "launches": launches,
                      "proposals_evaluated_per_launch": st[0] / launches,
+                     "proposals_integrated_per_launch": st[4] / float(npts) / launches,
                      "steps_decided_per_launch": st[3] / launches,
                      "alg_flops_per_launch": flops_per_launch,
                      "fallback_steps_per_launch": st[1] / launches},

@@ -1069,7 +1069,7 @@ constexpr int kGraphSteps = 16;
 // Speculation depth chosen when the descriptor leaves it to the library:
 // enough slots that one launch integrates about 2^20 points.
 constexpr int64_t kSpecPoints = 1 << 20;
-constexpr int kMaxSpec = 32;
+constexpr int kMaxSpec = 64;  // slots per chain and launch (a 64-bit mask tracks them)
 
 struct cpab_reg2d {
   cpab_ctx* ctx = nullptr;
@@ -1205,12 +1205,13 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   }
   // the re-arming workspace (normals and proposals of every slot) must fit in LDS
   while (K > 1 && (size_t)mcmc_arm_doubles(K, q.d) * sizeof(double) > (size_t)ctx->max_lds) K--;
-  // room for twice the library's own choice, for later tuning of the depth
+  // room for four times the library's own choice, for later tuning of the
+  // depth (with early abort, deep speculation costs little when it fails)
   int K_cap = K;
   {
-    const int64_t k2 = 2 * kSpecPoints / (nPts * (int64_t)q.nChains);
-    const int auto2 = k2 < 1 ? 1 : (k2 > kMaxSpec ? kMaxSpec : (int)k2);
-    if (auto2 > K_cap) K_cap = auto2;
+    const int64_t k4 = 4 * kSpecPoints / (nPts * (int64_t)q.nChains);
+    const int auto4 = k4 < 1 ? 1 : (k4 > kMaxSpec ? kMaxSpec : (int)k4);
+    if (auto4 > K_cap) K_cap = auto4;
     while (K_cap > K && (size_t)mcmc_arm_doubles(K_cap, q.d) * sizeof(double) > (size_t)ctx->max_lds) K_cap--;
   }
   if ((size_t)(2 * q.nC * 6) * sizeof(double) > (size_t)ctx->max_lds ||
@@ -1280,15 +1281,17 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
     return fail(ctx, CPAB_ERR_ARG, "image too large for the launch grid");
   }
   r->K_cap = K_cap;
-  // scratch, zeroed once: theta_prop | ll_prop | tables | partial | partial_nfb | slot_res | lz | Bt |
-  // done_slot | stop | done (in doubles; done_slot starts on a 128-byte boundary)
+  // scratch, zeroed once: theta_prop | ll_prop | tables | partial | partial_nfb | partial_px | slot_res |
+  // lz | Bt | done_slot | stop | done | order | ready, abort_after (in doubles; done_slot starts on a
+  // 128-byte boundary)
   const size_t nck = (size_t)q.nChains * K_cap;
   const size_t nB = (size_t)q.nC * 6 * q.d;
   const size_t o_ds =
-      (nck * q.d + q.nChains + nck * 2 * q.nC * 6 + 2 * nck * D.nBlocks + 2 * nck + 2 * nck * q.d + nB + 15) / 16 * 16;
+      (nck * q.d + q.nChains + nck * 2 * q.nC * 6 + 3 * nck * D.nBlocks + 3 * nck + 2 * nck * q.d + nB + 15) / 16 * 16;
   const size_t n_ds = nck * DONE_STRIDE / 2;
   const size_t n_ord = ((size_t)q.nChains * D.nBlocks + 1) / 2;  // ints, in doubles
-  const size_t n = o_ds + n_ds + q.nChains + q.nChains + n_ord;
+  const size_t o_ctl = o_ds + n_ds + 2 * q.nChains + n_ord;
+  const size_t n = o_ctl + q.nChains + (q.nChains + 1) / 2;  // ready (64-bit) and abort_after (int) per chain
   hipError_t e = hipMalloc(&r->scratch, n * sizeof(double));
   // zeroed on the context's stream, so the kernels after it see zeros
   if (e == hipSuccess) e = hipMemsetAsync(r->scratch, 0, n * sizeof(double), ctx->stream);
@@ -1302,8 +1305,9 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   D.tables = D.ll_prop + q.nChains;
   D.partial = D.tables + nck * 2 * q.nC * 6;
   D.partial_nfb = D.partial + nck * D.nBlocks;
-  D.slot_res = D.partial_nfb + nck * D.nBlocks;
-  D.lz = D.slot_res + 2 * nck;
+  D.partial_px = D.partial_nfb + nck * D.nBlocks;
+  D.slot_res = D.partial_px + nck * D.nBlocks;
+  D.lz = D.slot_res + 3 * nck;
   double* Bt = D.lz + 2 * nck * q.d;
   D.Bt = Bt;
   D.done_slot = reinterpret_cast<unsigned int*>(r->scratch + o_ds);
@@ -1311,7 +1315,19 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   D.done = reinterpret_cast<unsigned int*>(D.stop + q.nChains);
   // blocks in dispatch order by their work when lanes take several pixels
   D.order = D.ptsPerWave > 64 ? reinterpret_cast<int*>(r->scratch + o_ds + n_ds + 2 * q.nChains) : nullptr;
+  D.ready = reinterpret_cast<unsigned long long*>(r->scratch + o_ctl);
+  D.abort_after = reinterpret_cast<int*>(D.ready + q.nChains);
   r->host_step.resize(q.nChains);
+  {  // no slot is aborted before the first launch
+    const std::vector<int> none(q.nChains, kNoAbort);
+    e = hipMemcpyAsync(D.abort_after, none.data(), sizeof(int) * q.nChains, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      (void)hipFree(r->scratch);
+      delete r;
+      return fail(ctx, CPAB_ERR_HIP, "hipMemcpyAsync: %s", hipGetErrorString(e));
+    }
+  }
   // B is read transposed by the tables kernel (a copy: later changes of B are not seen)
   hipLaunchKernelGGL(k_transpose, dim3((unsigned)((nB + 255) / 256)), dim3(256), 0, ctx->stream, q.B, Bt,
                      q.nC * 6, q.d);

@@ -75,6 +75,7 @@ constexpr uint32_t SLOT_ACCEPT = 0x80000000u;
 // Spacing of the per-slot completion counters (unsigned ints): one 128-byte
 // line per counter.
 constexpr int DONE_STRIDE = 32;
+constexpr int kNoAbort = 0x7fffffff;
 
 // Threads per block of the registration step: 256 (4 waves) or 64.  A block
 // keeps its CU resources until its slowest wave ends, so small blocks free
@@ -105,7 +106,11 @@ struct McmcDesc {
   int64_t* stop;        // (nChains,) step counter at which the current run ends
   double* tables;       // (nChains, K, 2, nC, 6): per slot the Trels then the signed As
   double* partial;      // (nChains, K, nBlocks) block sums of squared errors
-  double* slot_res;     // (nChains, K, 2): ll' and rejected Trel proposals of each evaluated slot
+  double* partial_px;   // (nChains, K, nBlocks) pixels each block integrated (0: not started)
+  double* slot_res;     // (nChains, K, 3): ll', rejected Trel proposals and pixels of each slot
+  unsigned long long* ready;  // (nChains,) bit m: slot m's ll' is published (current launch; 0 between)
+  int* abort_after;     // (nChains,) a slot above it is not started: some slot at or below it
+                        // was found rejected on the all-accepted path (kNoAbort between launches)
   double* lz;           // (nChains, 2K, d): L z_{step+j}, j < 2K, for the current step counter
   double* partial_nfb;  // (nChains, K, nBlocks) block counts of rejected Trel proposals
   unsigned int* done;   // (nChains,) slots whose blocks have all finished in the current launch
@@ -113,8 +118,9 @@ struct McmcDesc {
                             // launch, one 128-byte line each (all 0 between launches)
   double* trace;        // (nChains, trace_len) or null
   unsigned long long* nfb;  // (nChains,) rejected Trel proposals of decided steps, accumulated; or null
-  int64_t* stats;       // (nChains, 4) or null: proposals evaluated, their rejected Trel
-                        // proposals, launches, steps decided
+  int64_t* stats;       // (nChains, 5) or null: proposals evaluated in full, rejected Trel
+                        // proposals of every pixel integrated, launches, steps decided,
+                        // pixels integrated
   int nBlocks;          // integration blocks per chain and slot
   int ptsPerWave;
   int tile8;            // 1: pixels are taken in 8 x 8 tiles (W and H multiples of 8)
@@ -123,9 +129,9 @@ struct McmcDesc {
   Geom g;
 };
 
-// LDS of the last block's epilogue (doubles): ll', fallback count and
-// acceptance uniform per slot.
-__host__ __device__ inline int mcmc_finish_doubles(int K) { return 3 * K; }
+// LDS of the last block's epilogue (doubles): ll', fallback count,
+// acceptance uniform and pixel count per slot.
+__host__ __device__ inline int mcmc_finish_doubles(int K) { return 4 * K; }
 // LDS of k_mcmc_prep (doubles): normals and proposals of every slot.
 __host__ __device__ inline int mcmc_arm_doubles(int K, int d) { return 2 * K * d; }
 // LDS of k_mcmc_tables (doubles): one proposal and its Avees, two normals, or
@@ -338,10 +344,12 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
   double* llp = w;
   double* nfs = w + K;
   double* lus = nfs + K;
+  double* pxs = lus + K;
   for (int m = threadIdx.x; m < kact; m += blockDim.x) {
-    const double* r = D.slot_res + ((int64_t)ch * K + m) * 2;
+    const double* r = D.slot_res + ((int64_t)ch * K + m) * 3;
     llp[m] = __hip_atomic_load(r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     nfs[m] = __hip_atomic_load(r + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    pxs[m] = __hip_atomic_load(r + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // log of the slot's acceptance uniform (read only if its ll does not improve)
     if (!init) {
       const int64_t s = step + m;
@@ -355,8 +363,11 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
     double ll = D.ll[ch];
     int nd = 0, la = -1;
     int64_t nacc = 0;
-    double nf_dec = 0.0, nf_all = 0.0;
-    for (int m = 0; m < kact; m++) nf_all += nfs[m];
+    double nf_dec = 0.0, nf_all = 0.0, px_all = 0.0;
+    for (int m = 0; m < kact; m++) {
+      nf_all += nfs[m];
+      px_all += pxs[m];
+    }
     for (int m = 0; m < kact; m++) {
       bool acc;
       if (init) {
@@ -387,14 +398,19 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
       D.step[ch] = step + nd;
     }
     if (D.nfb) D.nfb[ch] += (unsigned long long)nf_dec;
+    const int ab = __hip_atomic_load(D.abort_after + ch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (D.stats) {
-      int64_t* st = D.stats + (int64_t)ch * 4;
-      st[0] += kact;
+      int64_t* st = D.stats + (int64_t)ch * 5;
+      st[0] += ab < kact ? ab + 1 : kact;  // slots at or below abort_after all ran in full
       st[1] += (int64_t)nf_all;
       st[2] += 1;
       st[3] += init ? 0 : nd;
+      st[4] += (int64_t)px_all;
     }
-    D.done[ch] = 0;  // re-arm for the next launch (kernel boundaries order it)
+    // re-arm for the next launch (kernel boundaries order it)
+    D.done[ch] = 0;
+    D.ready[ch] = 0ull;
+    D.abort_after[ch] = kNoAbort;
     sh[0] = la;
     sh[1] = init ? 0 : nd;
   }
@@ -423,16 +439,34 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
   }
 }
 
+// The test of slot j on the all-accepted path (slots 0..j-1 accepted, so the
+// state is ll'_{j-1}, or the chain's ll for j = 0): Metropolis.py:64-82.
+__device__ inline bool mcmc_accepts(const McmcDesc& D, int ch, int64_t step, int j, double ll_prev, double ll_j) {
+  const double lr = ll_j - ll_prev;
+  if (lr > 0) return true;
+  const int64_t s = step + j;
+  const Philox u = philox4x32_10((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)ch, SLOT_ACCEPT, D.seed_lo, D.seed_hi);
+  const double lu = log(u53(u.x0, u.x1));
+  return D.inverted ? (lu > lr) : (lu < lr);
+}
+
 // One launch evaluates up to K proposals of every chain: grid (K * nBlocks,
-// nChains), block b integrates pixel range b / K of slot b % K, so the
-// blocks of one pixel range (the slowest one holds the domain corner) are
-// dispatched together.  A wave's 64 pixels are an 8 x 8 tile when the image
-// allows: neighbouring pixels flow alike and cross facets at about the same
-// steps, so the lanes of a wave need similar numbers of trips (one chain,
-// 20 slots: +11 % over 64 pixels of a row, measured on MI355X).  init == 1:
-// evaluate the armed (noise-free) slot 0 as the current state.  A chain
-// whose run has ended (step >= stop) is left as it is.  Every block of a live chain, evaluating or not, counts its arrival,
-// so the last arrival is the only block that touches the chain's state.
+// nChains).  Dispatch order: the first pixel block of every slot (it holds
+// the domain corner, whose pixel falls back at every step: the longest
+// chain of trips), then the slots one after another.  Slot m is accepted on
+// the all-accepted path or not as soon as slots m-1 and m are summed, so a
+// rejection is usually known while later slots have not started: their
+// blocks then skip their pixels (early abort; every slot at or below the
+// first rejection always runs in full, so the chain does not depend on the
+// timing -- only the counts of work done do).  A wave's 64 pixels are an
+// 8 x 8 tile when the image allows: neighbouring pixels flow alike and cross
+// facets at about the same steps, so the lanes of a wave need similar
+// numbers of trips (one chain, 20 slots: +11 % over 64 pixels of a row,
+// measured on MI355X).  init == 1: evaluate the armed (noise-free) slot 0 as
+// the current state.  A chain whose run has ended (step >= stop) is left as
+// it is.  Every block of a live chain, evaluating or not, counts its
+// arrival, so the last arrival is the only block that touches the chain's
+// state.
 template <int TESS, bool POW2, int BLOCK, bool FAST>
 __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
   extern __shared__ double lds[];
@@ -441,9 +475,17 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
   __shared__ int fin[2];
   const int ch = blockIdx.y;
   const int K = D.K;
-  const int slot = (int)(blockIdx.x % (unsigned)K);
-  const int bx = D.order ? D.order[(int64_t)ch * D.nBlocks + (int)(blockIdx.x / (unsigned)K)]
-                         : (int)(blockIdx.x / (unsigned)K);
+  const int nB = D.nBlocks;
+  int slot, pos;  // pos: the block's place in its slot's sequence of pixel blocks
+  if ((int)blockIdx.x < K) {
+    slot = (int)blockIdx.x;
+    pos = 0;
+  } else {
+    const int r = (int)blockIdx.x - K;
+    slot = r / (nB - 1);
+    pos = 1 + r - slot * (nB - 1);
+  }
+  const int bx = D.order ? D.order[(int64_t)ch * nB + pos] : pos;
   const int nC = D.nC;
   const int64_t step = D.step[ch];
   int kact = 1;
@@ -452,9 +494,17 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
     if (left <= 0) return;  // grid-uniform per chain: no block of this chain counts
     kact = left < K ? (int)left : K;
   }
+  // whether this block integrates: read once by thread 0 (abort_after may
+  // change under the block) so the decision, and the barriers after it, are
+  // block-uniform
+  if (threadIdx.x == 0)
+    fin[0] = slot < kact &&
+             (init || slot <= __hip_atomic_load(D.abort_after + ch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  __syncthreads();
+  const bool run = fin[0] != 0;
   double acc = 0.0;
   int nfb = 0;
-  if (slot < kact) {
+  if (run) {
     const double* tab = D.tables + ((int64_t)ch * K + slot) * 2 * nC * 6;
     for (int i = threadIdx.x; i < 2 * nC * 6; i += blockDim.x) lds[i] = tab[i];
     __syncthreads();
@@ -535,15 +585,20 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
   __syncthreads();
   unsigned int* ds = D.done_slot + ((int64_t)ch * K + slot) * DONE_STRIDE;
   if (threadIdx.x == 0) {
-    if (slot < kact) {
+    if (slot < kact) {  // a block that did not start stores zeros
       double sum = 0.0, sumnf = 0.0;
       for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
         sum += red[i];
         sumnf += rednf[i];
       }
-      const int64_t at = ((int64_t)ch * K + slot) * D.nBlocks + bx;
+      const int64_t nPts = (int64_t)D.H * D.W;
+      const int64_t p0 = (int64_t)bx * (blockDim.x >> 6) * D.ptsPerWave;
+      const int64_t p1 = p0 + (int64_t)(blockDim.x >> 6) * D.ptsPerWave;
+      const double px = run ? (double)((p1 < nPts ? p1 : nPts) - (p0 < nPts ? p0 : nPts)) : 0.0;
+      const int64_t at = ((int64_t)ch * K + slot) * nB + bx;
       __hip_atomic_store(D.partial + at, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(D.partial_nfb + at, sumnf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(D.partial_px + at, px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     const unsigned int prev = __hip_atomic_fetch_add(ds, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     slot_last = prev == (unsigned int)(D.nBlocks - 1);
@@ -556,24 +611,50 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
   if (!slot_last) return;  // block-uniform
   if (slot < kact && threadIdx.x < 64) {  // the first wave sums the slot's block sums, lane-strided
     const int lane = threadIdx.x;
-    const double* ps = D.partial + ((int64_t)ch * K + slot) * D.nBlocks;
-    const double* pn = D.partial_nfb + ((int64_t)ch * K + slot) * D.nBlocks;
-    double s = 0.0, f = 0.0;
-    for (int i = lane; i < D.nBlocks; i += 64) {
+    const double* ps = D.partial + ((int64_t)ch * K + slot) * nB;
+    const double* pn = D.partial_nfb + ((int64_t)ch * K + slot) * nB;
+    const double* pp = D.partial_px + ((int64_t)ch * K + slot) * nB;
+    double s = 0.0, f = 0.0, x = 0.0;
+    for (int i = lane; i < nB; i += 64) {
       s += __hip_atomic_load(ps + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       f += __hip_atomic_load(pn + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      x += __hip_atomic_load(pp + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       s += __shfl_xor(s, o, 64);
       f += __shfl_xor(f, o, 64);
+      x += __shfl_xor(x, o, 64);
     }
-    if (lane == 0) {  // thread 0: its release below publishes these
+    if (lane == 0) {  // thread 0: its releases below publish these
       double l = s * (-0.5 / (D.sigma * D.sigma));
       if (D.use_ave_ll) l = l / (double)((int64_t)D.H * D.W);
-      double* r = D.slot_res + ((int64_t)ch * K + slot) * 2;
+      double* r = D.slot_res + ((int64_t)ch * K + slot) * 3;
       __hip_atomic_store(r, l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(r + 1, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(r + 2, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (!init && kact > 1) {
+        // Publish the slot, then test every slot j in {slot, slot + 1} whose
+        // own sum and predecessor's are both published: of two neighbours
+        // the later to publish sees both.  A rejection stops the slots above
+        // it from starting.  A slot that did not run in full has a partial
+        // sum, but it lies above the first rejection, which is always found
+        // (its slot and the one before it run in full), so its test can only
+        // lower the bound to a value at or above the first rejection.
+        const unsigned long long mask =
+            __hip_atomic_fetch_or(D.ready + ch, 1ull << slot, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) |
+            (1ull << slot);
+        for (int j = slot; j <= slot + 1 && j < kact; j++) {
+          if (!((mask >> j) & 1ull) || (j > 0 && !((mask >> (j - 1)) & 1ull))) continue;
+          const double lj = __hip_atomic_load(D.slot_res + ((int64_t)ch * K + j) * 3, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+          const double lp = j == 0 ? D.ll[ch]
+                                   : __hip_atomic_load(D.slot_res + ((int64_t)ch * K + j - 1) * 3, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+          if (!mcmc_accepts(D, ch, step, j, lp, lj))
+            __hip_atomic_fetch_min(D.abort_after + ch, j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
     }
   }
   if (threadIdx.x == 0) {

@@ -120,8 +120,9 @@ class DeviceMetropolis:
         self.trace_len = trace_len
         self.trace = torch.zeros((nChains, max(trace_len, 1)), dtype=f64, device=dev)
         self.n_fallback = torch.zeros(nChains, dtype=torch.int64, device=dev)
-        # proposals evaluated, their rejected Trel proposals, launches, steps decided
-        self.stats = torch.zeros((nChains, 4), dtype=torch.int64, device=dev)
+        # proposals evaluated in full, rejected Trel proposals of every pixel integrated,
+        # launches, steps decided, pixels integrated
+        self.stats = torch.zeros((nChains, 5), dtype=torch.int64, device=dev)
         p = problem.params
         T = sp.tessellation
         desc = RegDesc(nChains, d, T.type, sp.nC, int(T.nCs[0]), int(T.nCs[1]), float(T.incs[0]),
@@ -174,7 +175,7 @@ class DeviceMetropolis:
         check(lib().cpab_reg2d_set_spec_depth(self._h, int(K)), self._ctx.handle)
         self.spec_depth = int(K)
 
-    def autotune_spec(self, candidates=(8, 12, 16, 20, 24, 32), steps=192, use_graph=True):
+    def autotune_spec(self, candidates=(8, 16, 24, 32, 40, 48), steps=192, use_graph=True):
         """Time `steps` Metropolis steps of these chains at each candidate
         speculation depth (the steps are real steps of the chains) and keep
         the fastest.  Returns {depth: steps per second}."""

@@ -269,12 +269,17 @@ typedef struct {
   uint64_t *n_fallback; /* optional (nChains,): accumulates the integration steps
                            whose Trel proposal left its cell (RK2 redone), over the
                            decided steps */
-  int spec_depth;       /* proposals evaluated per chain and launch: 1 = one step per
-                           launch; K > 1 = speculative prefetch of the next K steps on
-                           the all-accepted path (same chain bit for bit); 0 = chosen
-                           by the library (about 2^20 points per launch, at most 16) */
-  int64_t *stats;       /* optional (nChains, 4), accumulated: proposals evaluated,
-                           their rejected Trel proposals, launches, steps decided */
+  int spec_depth;       /* proposals evaluated per chain and launch, at most 64: 1 = one
+                           step per launch; K > 1 = speculative prefetch of the next K
+                           steps on the all-accepted path (same chain bit for bit; slots
+                           above a rejection found during the launch are not started);
+                           0 = chosen by the library (about 2^20 points per launch, at
+                           most 16) */
+  int64_t *stats;       /* optional (nChains, 5), accumulated: proposals evaluated in
+                           full, rejected Trel proposals of every pixel integrated,
+                           launches, steps decided, pixels integrated (with speculation a
+                           slot above a known rejection is not started, so the counts of
+                           work done depend on timing; the chain does not) */
   int pts_per_wave;     /* pixels a wave integrates (a multiple of 64, at most 4096);
                            0 = chosen by the library from the launch size (64, 128 or
                            256 pixels per wave; many chains one step per launch also
@@ -304,8 +309,8 @@ int cpab_reg2d_run(cpab_reg2d *r, int nSteps, int use_graph);
 int cpab_reg2d_last_proposal(cpab_reg2d *r, double *theta_prop, double *ll_prop);
 /* The speculation depth in use (spec_depth as resolved at create). */
 int cpab_reg2d_spec_depth(cpab_reg2d *r, int *K);
-/* Change the speculation depth between runs (up to twice the library's own
- * choice at create, or the depth asked for if larger); the armed proposals
+/* Change the speculation depth between runs (up to four times the library's
+ * own choice at create, at most 64, or the depth asked for if larger); the armed proposals
  * are recomputed for the new depth, so the chain is unchanged.  Call after
  * cpab_reg2d_init.  Synchronises the context's stream. */
 int cpab_reg2d_set_spec_depth(cpab_reg2d *r, int K);
