@@ -369,7 +369,7 @@ def main():
         return out
 
     log("bench: warmup %d steps, then the timed %d-step chain (speculative prefetch)" % (args.warmup, args.steps))
-    spec = chain(0, args.warmup, args.steps, True, tune=(12, 16, 24, 32, 40, 48))
+    spec = chain(0, args.warmup, args.steps, True, tune=(16, 24, 32, 40, 48, 56, 64))
     seq = None
     if not args.no_seq:
         log("bench: the same chain one step per launch")

@@ -1114,10 +1114,15 @@ int reg2d_launch_step(cpab_reg2d* r, hipStream_t s, int init) {
     int rc = prof_begin(r->ctx, &a, &b);
     if (rc) return rc;
   }
-#define STEP_LAUNCH(T_, P_)                                                                            \
-  do {                                                                                                 \
-    if (r->block == 64) hipLaunchKernelGGL((k_mcmc_step<T_, P_, 64, true>), grid, dim3(64), sh, s, D, init); \
-    else hipLaunchKernelGGL((k_mcmc_step<T_, P_, 256, false>), grid, dim3(256), sh, s, D, init);          \
+  // the fast lookup pays once the launch holds four waves per SIMD or more
+  // (a lone one-step chain is latency-bound: one wave per SIMD)
+  const int64_t waves = (int64_t)grid.x * grid.y * (r->block / 64);
+  const bool fast = waves >= (int64_t)r->ctx->n_cu * 4 * 4;
+#define STEP_LAUNCH(T_, P_)                                                                                   \
+  do {                                                                                                        \
+    if (r->block == 64) hipLaunchKernelGGL((k_mcmc_step<T_, P_, 64, true>), grid, dim3(64), sh, s, D, init);  \
+    else if (fast) hipLaunchKernelGGL((k_mcmc_step<T_, P_, 256, true>), grid, dim3(256), sh, s, D, init);     \
+    else hipLaunchKernelGGL((k_mcmc_step<T_, P_, 256, false>), grid, dim3(256), sh, s, D, init);              \
   } while (0)
   if (D.tess == TESS_I) {
     if (r->pow2) STEP_LAUNCH(TESS_I, true);

@@ -118,10 +118,10 @@ __device__ __forceinline__ int cell1(const double* p, const Geom& g) {
 // the index mins and the min/max operand canonicalisations IEEE mode needs.
 // The test is wave-uniform: a wave with a lane on or past the boundary (or
 // NaN) runs the general lookup only.  Measured on MI355X the gain depends
-// on the launch: 256 registration chains +14-17 %, the 512x512 calc_T +3 %,
-// the 1 024-instance batch +1.5 %; but a 65536-point calc_T -28 %, one
-// registration chain -4 % and the serving mix's work items -5 %.  So each
-// launch chooses it.
+// on the launch: 256 registration chains +14-17 %, one chain with deep
+// speculation +1-2 %, the 512x512 calc_T +3 %, the 1 024-instance batch
+// +1.5 %; but a 65536-point calc_T -28 % and the serving mix's work items
+// -5 %.  So each launch chooses it.
 template <int TESS, bool POW2, bool FAST = false>
 __device__ __forceinline__ int cell2(const double* p, const Geom& g) {
   if (TESS == TESS_II) {

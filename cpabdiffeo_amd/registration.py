@@ -175,7 +175,7 @@ class DeviceMetropolis:
         check(lib().cpab_reg2d_set_spec_depth(self._h, int(K)), self._ctx.handle)
         self.spec_depth = int(K)
 
-    def autotune_spec(self, candidates=(8, 16, 24, 32, 40, 48), steps=192, use_graph=True):
+    def autotune_spec(self, candidates=(8, 16, 24, 32, 40, 48, 56, 64), steps=192, use_graph=True):
         """Time `steps` Metropolis steps of these chains at each candidate
         speculation depth (the steps are real steps of the chains) and keep
         the fastest.  Returns {depth: steps per second}."""
