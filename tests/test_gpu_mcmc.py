@@ -124,6 +124,7 @@ def test_speculation_is_the_same_chain(problem, gpu, nChains, spec, graph):
         torch.cuda.synchronize()
         out.append([x.cpu().numpy().copy() for x in (mh.theta, mh.trace, mh.n_accepted, mh.n_fallback,
                                                        mh.step_counter, mh.stats)])
+        depth = mh.spec_depth
         mh.close()
     a, b = out
     for i in range(5):
@@ -131,7 +132,10 @@ def test_speculation_is_the_same_chain(problem, gpu, nChains, spec, graph):
     assert (a[4] == 60).all()
     st = b[5]
     assert (st[:, 3] == 60).all()  # steps decided
-    assert (st[:, 0] >= 60).all()  # proposals evaluated
+    assert (st[:, 0] >= 60).all()  # proposals evaluated in full
+    npx = p.n * p.n
+    assert (st[:, 4] >= st[:, 0] * npx).all()  # pixels integrated: at least those of the full proposals
+    assert (st[:, 4] <= st[:, 2] * depth * npx).all()  # and never more than the slots launched
     assert 0 < a[2].min() and a[2].max() < 60  # accepts and rejects both happened
 
 
