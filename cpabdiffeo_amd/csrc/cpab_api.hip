@@ -1248,7 +1248,6 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   D.trace_len = q.trace ? q.trace_len : 0;
   D.seed_lo = (uint32_t)q.seed;
   D.seed_hi = (uint32_t)(q.seed >> 32);
-  D.B = q.B;
   D.L = q.Lprop;
   D.src = q.src;
   D.dst = q.dst;

@@ -19,17 +19,20 @@
 //             log(u) > log_ratio, Metropolis.py:77-82), slot after slot until
 //             the first rejection, advances the step counter by the slots
 //             decided, and arms the next K proposals (theta') for the chain.
-//   then      k_mcmc_tables, a launch of (2K x chains) blocks, forms the
-//             armed proposals' per-cell tables (B theta', expm) for the next
-//             step launch and the L z vectors its last block will arm from.
+//   then      k_mcmc_tables, a launch of (2K (+1) x chains) blocks, forms
+//             the armed proposals' per-cell tables (B theta', expm) for the
+//             next step launch, the L z vectors its last block will arm
+//             from, and (several pixels per lane) the next dispatch order.
 //
 // Speculation (K > 1, prefetching in the sense of Brockwell, J. Comput.
 // Graph. Stat. 15(1), 2006): slot m holds the proposal of step s + m on the
 // path where steps s .. s+m-1 are all accepted.  Slots are decided in order;
 // the first rejected slot ends the launch's steps and the later slots are
-// discarded.  Every decided step uses exactly the numbers the K = 1 chain
-// uses (same proposal sums, same random counters), so the chain is the same
-// bit for bit; only idle GPU capacity is spent on the discarded slots.
+// discarded -- and, once the rejection is known during the launch, the
+// blocks of later slots that have not started skip their pixels (early
+// abort).  Every decided step uses exactly the numbers the K = 1 chain uses
+// (same proposal sums, same random counters), so the chain is the same bit
+// for bit.
 // k_mcmc_prep (+ k_mcmc_tables) arms the first (noise-free) proposal before
 // the init launch.
 //
@@ -92,8 +95,7 @@ struct McmcDesc {
   int use_ave_ll, inverted, trace_len;
   uint32_t seed_lo, seed_hi;
   int K;                // proposals evaluated per chain and launch (speculation depth)
-  const double* B;      // (nC*6, d) row-major
-  const double* Bt;     // (d, nC*6) row-major: B transposed
+  const double* Bt;     // (d, nC*6) row-major: the basis B (Avees = B theta) transposed
   const double* L;      // (d, d) row-major lower triangular
   const float* src;     // (H, W)
   const float* dst;     // (H, W)
@@ -142,7 +144,8 @@ __host__ __device__ inline int mcmc_tables_doubles(int nC, int d, int nBlocks) {
   return n > o ? n : o;
 }
 
-// z_s ~ N(0, I) (d normals, Philox counter (s, chain, pair)) into z.
+// Normals 2t and 2t+1 of z_s ~ N(0, I_d) into z (Philox counter (s, chain,
+// t), Box-Muller).
 __device__ inline void mcmc_normal_pair(const McmcDesc& D, int ch, int64_t s, int t, double* z) {
   const int d = D.d;
   const Philox r = philox4x32_10((uint32_t)s, (uint32_t)(s >> 32), (uint32_t)ch, (uint32_t)t, D.seed_lo, D.seed_hi);
