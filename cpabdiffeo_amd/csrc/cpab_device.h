@@ -130,8 +130,12 @@ __device__ __forceinline__ int cell2(const double* p, const Geom& g) {
   }
   const double px = p[0], py = p[1];
   if (POW2 && FAST) {
-    const bool strict = (px > 0.0) & (px < g.top_x) & (py > 0.0) & (py < g.top_y);
-    if (__builtin_amdgcn_ballot_w64(!strict) == 0) {
+    // the strict-interior test as lane masks (fcmp predicates: 2 ordered
+    // greater, 4 ordered less; inactive lanes read 0), combined in scalar code
+    const uint64_t ok = __builtin_amdgcn_fcmp(px, 0.0, 2) & __builtin_amdgcn_fcmp(px, g.top_x, 4) &
+                        __builtin_amdgcn_fcmp(py, 0.0, 2) & __builtin_amdgcn_fcmp(py, g.top_y, 4);
+    const uint64_t ex = __builtin_amdgcn_read_exec();
+    if ((ok & ex) == ex) {
       const double sx = px * g.rinc_x, sy = py * g.rinc_y;
       const double qx = __builtin_trunc(sx), qy = __builtin_trunc(sy);
       const double x = sx - qx, y = sy - qy;

@@ -250,9 +250,15 @@ __device__ __forceinline__ bool trip(double* p, int& c, int& k, const double*& r
   double y[DIM];
   affine<DIM>(y, row, p);
   if (!fast) {  // RK2 midpoint substep (solveODE, calc_2Dto2D.cu:182-194)
+    // The reference's h * v / 2 is fl(h v) halved; (h / 2) * v rounds the
+    // same real number h v / 2, and halving commutes with rounding, so the
+    // two agree bit for bit unless |h v / 2| is subnormal (below 2^-1022).
+    // A nonzero v = A [p; 1] is at least ~2^-53 of its largest term, so that
+    // takes table entries near 1e-290.  One multiply per coordinate fewer.
+    const double hh = h * 0.5;
     double m[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; d++) m[d] = p[d] + h * y[d] / 2.;
+    for (int d = 0; d < DIM; d++) m[d] = p[d] + hh * y[d];
     affine<DIM>(y, row, m);
 #pragma unroll
     for (int d = 0; d < DIM; d++) y[d] = p[d] + y[d] * h;
