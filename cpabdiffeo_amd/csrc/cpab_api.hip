@@ -1070,6 +1070,7 @@ constexpr int kGraphSteps = 16;
 // enough slots that one launch integrates about 2^20 points.
 constexpr int64_t kSpecPoints = 1 << 20;
 constexpr int kMaxSpec = 64;  // slots per chain and launch (a 64-bit mask tracks them)
+constexpr int kAbortCheckpoints = 3;  // steps at which a one-pixel-per-lane wave polls for early abort
 
 struct cpab_reg2d {
   cpab_ctx* ctx = nullptr;
@@ -1261,6 +1262,7 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   D.K = K;
   D.g = g;
   D.tile8 = (q.W % 8 == 0 && q.H % 8 == 0) ? 1 : 0;
+  D.poll = kAbortCheckpoints;
   // One pixel per lane while a launch holds few waves per SIMD (latency
   // hiding needs the waves); with many (batched chains) 2 or 4 pixels per
   // lane with refill keep lanes busy past their slowest pixel (measured on

@@ -126,6 +126,7 @@ struct McmcDesc {
   int nBlocks;          // integration blocks per chain and slot
   int ptsPerWave;
   int tile8;            // 1: pixels are taken in 8 x 8 tiles (W and H multiples of 8)
+  int poll;             // abort checkpoints per pixel run (one pixel per lane only; 0: none)
   int* order;           // (nChains, nBlocks): the pixel block each dispatch position integrates,
                         // or null: dispatch position = pixel block
   Geom g;
@@ -408,7 +409,7 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
       st[1] += (int64_t)nf_all;
       st[2] += 1;
       st[3] += init ? 0 : nd;
-      st[4] += (int64_t)px_all;
+      st[4] += llround(px_all);
     }
     // re-arm for the next launch (kernel boundaries order it)
     D.done[ch] = 0;
@@ -459,9 +460,10 @@ __device__ inline bool mcmc_accepts(const McmcDesc& D, int ch, int64_t step, int
 // chain of trips), then the slots one after another.  Slot m is accepted on
 // the all-accepted path or not as soon as slots m-1 and m are summed, so a
 // rejection is usually known while later slots have not started: their
-// blocks then skip their pixels (early abort; every slot at or below the
-// first rejection always runs in full, so the chain does not depend on the
-// timing -- only the counts of work done do).  A wave's 64 pixels are an
+// blocks then skip their pixels, and waves already running stop at the next
+// of a few step checkpoints (early abort; every slot at or below the first
+// rejection always runs in full, so the chain does not depend on the timing
+// -- only the counts of work done do).  A wave's 64 pixels are an
 // 8 x 8 tile when the image allows: neighbouring pixels flow alike and cross
 // facets at about the same steps, so the lanes of a wave need similar
 // numbers of trips (one chain, 20 slots: +11 % over 64 pixels of a row,
@@ -473,7 +475,7 @@ __device__ inline bool mcmc_accepts(const McmcDesc& D, int ch, int64_t step, int
 template <int TESS, bool POW2, int BLOCK, bool FAST>
 __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
   extern __shared__ double lds[];
-  __shared__ double red[BLOCK / 64], rednf[BLOCK / 64];
+  __shared__ double red[BLOCK / 64], rednf[BLOCK / 64], redpx[BLOCK / 64];
   __shared__ int last, slot_last;
   __shared__ int fin[2];
   const int ch = blockIdx.y;
@@ -505,7 +507,7 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
              (init || slot <= __hip_atomic_load(D.abort_after + ch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   __syncthreads();
   const bool run = fin[0] != 0;
-  double acc = 0.0;
+  double acc = 0.0, pxw = 0.0;  // SSD, pixels' worth integrated
   int nfb = 0;
   if (run) {
     const double* tab = D.tables + ((int64_t)ch * K + slot) * 2 * nC * 6;
@@ -521,6 +523,13 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
     const int nT = D.nT, nS = D.nS;
     const double h = D.h;
     if (begin < nPts) {
+      // Abort checkpoints: with one pixel per lane every lane runs steps 0..nT
+      // once, so the wave can stop at fixed steps when its slot lies above a
+      // rejection found meanwhile (no cost per trip: the trip loop's bound
+      // is the next checkpoint instead of nT).
+      const int nCk = (!init && kact > 1 && D.ptsPerWave == 64 && nT > D.poll) ? D.poll : 0;
+      const int tstep = nCk > 0 ? nT / (nCk + 1) : nT;  // steps between checkpoints
+      int cp = 0, tb = tstep;                             // checkpoint index, trip loop bound
       int64_t next = begin;
       double p[2];
       int c = 0, t = 0, k = 0;
@@ -532,6 +541,7 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
           const float w = sample2(D.src, D.H, D.W, p[0], p[1]);
           const double e = (double)w - (double)D.dst[pix];
           acc += e * e;
+          pxw += 1.0;
           active = false;
         }
         const uint64_t idle = ballot(!active);
@@ -559,10 +569,22 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
           next += __popcll(idle);
         }
         if (ballot(active) == 0) break;
+        if (cp < nCk && ballot(active && t >= tb) != 0) {  // a lane reached checkpoint cp
+          const int ab = __builtin_amdgcn_readfirstlane(
+              __hip_atomic_load(D.abort_after + ch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+          if (slot > ab) {
+            if (active) pxw += (double)t / (double)nT;  // the wave stops: its slot is not needed
+            break;
+          }
+          do {
+            cp++;
+            tb = cp < nCk ? tb + tstep : nT;
+          } while (cp < nCk && ballot(active && t >= tb) != 0);
+        }
         if (!active || t >= nT) continue;
         do {
           if (trip<2, TESS, POW2, FAST>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) t++;
-        } while (ballot(t >= nT) == 0);
+        } while (ballot(t >= tb) == 0);
       }
     }
   }
@@ -580,24 +602,23 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
   for (int o = 32; o > 0; o >>= 1) {
     acc += __shfl_xor(acc, o, 64);
     nf += __shfl_xor(nf, o, 64);
+    pxw += __shfl_xor(pxw, o, 64);
   }
   if ((threadIdx.x & 63) == 0) {
     red[threadIdx.x >> 6] = acc;
     rednf[threadIdx.x >> 6] = nf;
+    redpx[threadIdx.x >> 6] = pxw;
   }
   __syncthreads();
   unsigned int* ds = D.done_slot + ((int64_t)ch * K + slot) * DONE_STRIDE;
   if (threadIdx.x == 0) {
     if (slot < kact) {  // a block that did not start stores zeros
-      double sum = 0.0, sumnf = 0.0;
+      double sum = 0.0, sumnf = 0.0, px = 0.0;
       for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
         sum += red[i];
         sumnf += rednf[i];
+        px += redpx[i];
       }
-      const int64_t nPts = (int64_t)D.H * D.W;
-      const int64_t p0 = (int64_t)bx * (blockDim.x >> 6) * D.ptsPerWave;
-      const int64_t p1 = p0 + (int64_t)(blockDim.x >> 6) * D.ptsPerWave;
-      const double px = run ? (double)((p1 < nPts ? p1 : nPts) - (p0 < nPts ? p0 : nPts)) : 0.0;
       const int64_t at = ((int64_t)ch * K + slot) * nB + bx;
       __hip_atomic_store(D.partial + at, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(D.partial_nfb + at, sumnf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

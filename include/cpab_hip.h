@@ -272,14 +272,17 @@ typedef struct {
   int spec_depth;       /* proposals evaluated per chain and launch, at most 64: 1 = one
                            step per launch; K > 1 = speculative prefetch of the next K
                            steps on the all-accepted path (same chain bit for bit; slots
-                           above a rejection found during the launch are not started);
+                           above a rejection found during the launch are not started,
+                           or stop at a step checkpoint);
                            0 = chosen by the library (about 2^20 points per launch, at
                            most 16) */
   int64_t *stats;       /* optional (nChains, 5), accumulated: proposals evaluated in
                            full, rejected Trel proposals of every pixel integrated,
-                           launches, steps decided, pixels integrated (with speculation a
-                           slot above a known rejection is not started, so the counts of
-                           work done depend on timing; the chain does not) */
+                           launches, steps decided, pixels' worth integrated (a pixel
+                           stopped at step t of nT counts t / nT; with speculation a
+                           slot above a known rejection is not started or stops early,
+                           so the counts of work done depend on timing; the chain
+                           does not) */
   int pts_per_wave;     /* pixels a wave integrates (a multiple of 64, at most 4096);
                            0 = chosen by the library from the launch size (64, 128 or
                            256 pixels per wave; many chains one step per launch also
