@@ -169,6 +169,34 @@ __device__ __forceinline__ int cell2(const double* p, const Geom& g) {
   return c + ((omx < y) ? 1 : 0);
 }
 
+// cell2's general type I path without branches: the inside and outside
+// classifications are both computed and one is selected, so a lane that is
+// alone in its wave (a single dependent chain, see lone_trips) pays no
+// branch and no strict-interior test.  Same result as cell2 for every point.
+template <bool POW2>
+__device__ __forceinline__ int cell2_flat(const double* p, const Geom& g) {
+  const double px = p[0], py = p[1];
+  const double p0 = dmin(g.top_x, dmax(0.0, px));
+  const double p1 = dmin(g.top_y, dmax(0.0, py));
+  double x, y;
+  const int i = axis_simplex<POW2>(p0, g.nC0, g.inc_x, g.rinc_x, x);
+  const int j = axis_simplex<POW2>(p1, g.nC1, g.inc_y, g.rinc_y, y);
+  const int c = (i + mul24(j, g.nC0)) * 4;
+  const bool left = px <= 0.0, right = px >= g.wall_x, up = py <= 0.0, down = py >= g.wall_y;
+  const double nC0 = (double)g.nC0, nC1 = (double)g.nC1;
+  const double sx = divinc<POW2>(px, g.inc_x, g.rinc_x), sy = divinc<POW2>(py, g.inc_y, g.rinc_y);
+  const double nx = divinc<POW2>(-px, g.inc_x, g.rinc_x), ny = divinc<POW2>(-py, g.inc_y, g.rinc_y);
+  const bool upL = up & (sy < sx), dnL = down & (sy - nC1 > nx);
+  const bool upR = up & (ny > sx - nC0), dnR = down & (sy - nC1 > sx - nC0);
+  const int tl = upL ? 0 : (dnL ? 2 : 3);
+  const int tr = upR ? 0 : (dnR ? 2 : 1);
+  const int tm = up ? 0 : 2;
+  const int tout = left ? tl : (right ? tr : tm);
+  const double omx = 1.0 - x;
+  const int tin = (x < y) ? ((omx < y) ? 2 : 3) : ((omx < y) ? 1 : 0);
+  return c + ((left | right | up | down) ? tout : tin);
+}
+
 // ------------------------------------------------------------------ 3D
 // calc_3Dto3D.cu:43-137
 template <int TESS, bool POW2>

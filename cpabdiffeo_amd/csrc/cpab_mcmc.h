@@ -582,6 +582,10 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
           } while (cp < nCk && ballot(active && t >= tb) != 0);
         }
         if (!active || t >= nT) continue;
+        if (__popcll(__builtin_amdgcn_read_exec()) == 1) {  // one lane left: it loops on its own
+          lone_trips<2, TESS, POW2, FAST>(p, c, k, t, tb, row, nfb, Ttab, Atab, h, nS, g);
+          continue;
+        }
         do {
           if (trip<2, TESS, POW2, FAST>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) t++;
         } while (ballot(t >= tb) == 0);
