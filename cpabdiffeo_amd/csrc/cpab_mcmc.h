@@ -455,9 +455,11 @@ __device__ inline bool mcmc_accepts(const McmcDesc& D, int ch, int64_t step, int
 }
 
 // One launch evaluates up to K proposals of every chain: grid (K * nBlocks,
-// nChains).  Dispatch order: the first pixel block of every slot (it holds
-// the domain corner, whose pixel falls back at every step: the longest
-// chain of trips), then the slots one after another.  Slot m is accepted on
+// nChains), or (nChains, K * nBlocks) for several chains.  Dispatch order:
+// the first pixel block of every slot (it holds the domain corner, whose
+// pixel falls back at most steps: the longest chain of trips), then the
+// slots one after another; with several chains, chain-fastest at each
+// place, so every chain's corner blocks start first.  Slot m is accepted on
 // the all-accepted path or not as soon as slots m-1 and m are summed, so a
 // rejection is usually known while later slots have not started: their
 // blocks then skip their pixels, and waves already running stop at the next
@@ -472,21 +474,24 @@ __device__ inline bool mcmc_accepts(const McmcDesc& D, int ch, int64_t step, int
 // it is.  Every block of a live chain, evaluating or not, counts its
 // arrival, so the last arrival is the only block that touches the chain's
 // state.
-template <int TESS, bool POW2, int BLOCK, bool FAST>
+template <int TESS, bool POW2, int BLOCK, bool FAST, bool SWAP>
 __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
   extern __shared__ double lds[];
   __shared__ double red[BLOCK / 64], rednf[BLOCK / 64], redpx[BLOCK / 64];
   __shared__ int last, slot_last;
   __shared__ int fin[2];
-  const int ch = blockIdx.y;
+  // SWAP: grid (chains, K x nBlocks), so the hardware, which dispatches x
+  // fastest, starts the first pixel block of every chain's slots first
+  const int ch = SWAP ? blockIdx.x : blockIdx.y;
+  const int bi = SWAP ? (int)blockIdx.y : (int)blockIdx.x;
   const int K = D.K;
   const int nB = D.nBlocks;
   int slot, pos;  // pos: the block's place in its slot's sequence of pixel blocks
-  if ((int)blockIdx.x < K) {
-    slot = (int)blockIdx.x;
+  if (bi < K) {
+    slot = bi;
     pos = 0;
   } else {
-    const int r = (int)blockIdx.x - K;
+    const int r = bi - K;
     slot = r / (nB - 1);
     pos = 1 + r - slot * (nB - 1);
   }
