@@ -402,6 +402,10 @@ __device__ __forceinline__ void integrate_range(int64_t begin, int64_t end, cons
     }
     if (ballot(active) == 0) break;
     if (!active || t >= nT) continue;
+    if (MODE == 0 && DIM == 2 && __popcll(__builtin_amdgcn_read_exec()) == 1) {  // one lane left: its own loop
+      lone_trips<DIM, TESS, POW2, FAST>(p, c, k, t, nT, row, nfb, Ttab, Atab, h, nS, g);
+      continue;
+    }
 #pragma unroll 2
     do {  // trips until some lane of the region finishes its point
       if (trip<DIM, TESS, POW2, FAST>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) {
