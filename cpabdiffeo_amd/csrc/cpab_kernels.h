@@ -307,7 +307,7 @@ __device__ __forceinline__ int lone_cell(const double* y, const Geom& g) {
   return cell_of<DIM, TESS, POW2>(y, g);
 }
 
-template <int DIM, int TESS, bool POW2, bool FAST = false>
+template <int DIM, int TESS, bool POW2>
 __device__ __forceinline__ void lone_trips(double* p, int& c, int& k, int& t, int tb, const double*& row, int& nfb,
                                            const double* Ttab, const double* Atab, double h, int nS,
                                            const Geom& g) {
@@ -403,7 +403,7 @@ __device__ __forceinline__ void integrate_range(int64_t begin, int64_t end, cons
     if (ballot(active) == 0) break;
     if (!active || t >= nT) continue;
     if (MODE == 0 && DIM == 2 && __popcll(__builtin_amdgcn_read_exec()) == 1) {  // one lane left: its own loop
-      lone_trips<DIM, TESS, POW2, FAST>(p, c, k, t, nT, row, nfb, Ttab, Atab, h, nS, g);
+      lone_trips<DIM, TESS, POW2>(p, c, k, t, nT, row, nfb, Ttab, Atab, h, nS, g);
       continue;
     }
 #pragma unroll 2

@@ -588,7 +588,7 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
         }
         if (!active || t >= nT) continue;
         if (__popcll(__builtin_amdgcn_read_exec()) == 1) {  // one lane left: it loops on its own
-          lone_trips<2, TESS, POW2, FAST>(p, c, k, t, tb, row, nfb, Ttab, Atab, h, nS, g);
+          lone_trips<2, TESS, POW2>(p, c, k, t, tb, row, nfb, Ttab, Atab, h, nS, g);
           continue;
         }
         do {
