@@ -7,23 +7,29 @@ Workload: 2D registration by Metropolis on a 256 x 256 FP32 blob image
 the squared-exponential CPA prior (4 x 4 rectangles x 4 triangles, nC = 64,
 zero normal boundary velocity, length scale 0.1, variance 1) plus
 N(0, 0.02^2) noise; Gaussian SSD likelihood with sigma = 0.02; random-walk
-proposals N(0, 0.01^2 Sigma_CPA).  One step = one proposal: theta' ->
-Avees -> expm tables -> calc_T_inv of the 65 536 pixel points (t = 1,
-100 steps x 10 RK2 substeps on facet crossings, FP64) -> bilinear remap
-(FP32) -> SSD -> Metropolis accept, on the GPU, launched from captured
-hipGraphs.  After W warmup steps the timed region is the K-step chain from
-theta = 0 with its counters reset, so the timed work does not depend on W.
+proposals N(0, 0.01^2 Sigma_CPA).  One Metropolis step = one proposal:
+theta' -> Avees -> expm tables -> calc_T_inv of the 65 536 pixel points
+(t = 1, 100 steps x 10 RK2 substeps on facet crossings, FP64) -> bilinear
+remap (FP32) -> SSD -> accept, on the GPU, launched from captured hipGraphs.
 
-Each launch evaluates the next K proposals on the all-accepted path and
-decides them in order (speculative prefetch): the chain is the same bit
-for bit as one step per launch, which is timed too and reported beside it.
-K is tuned on the workload itself before the timed run (each candidate
-runs the same chain from theta = 0).
+One bench step is one whole chain as configured: theta = 0, step counter
+and acceptance count reset, the log-likelihood of theta = 0 evaluated, then
+2 000 Metropolis steps.  W untimed chains run first, then K timed chains;
+every timed chain is the same chain (checked bit for bit).  value =
+2 000 / (seconds per chain).
+
+Each launch evaluates the next S proposals on the all-accepted path and
+decides them in order (speculative prefetch, S the speculation depth): the
+chain is the same bit for bit as one step per launch, which is timed too and
+reported beside it.  Before anything is timed every candidate depth runs a
+whole chain (its graph is captured then and kept), then each candidate is
+timed on a whole chain and the fastest is used.
 
 Prints one JSON line: proposals/s, the live roofline of the dominant kernel
-(algorithmic FP64 FLOPs of the proposals each launch evaluates, over the
-launch's HIP-event duration inside the timed region), and the CPU oracle
-timed on this host's cores.
+(algorithmic FP64 FLOPs of the decided proposals over the launches'
+HIP-event durations inside the timed region; the discarded speculative work
+is reported beside it), and the CPU oracle timed on this host's cores on
+proposals of the same chain.
 """
 import argparse
 import json
@@ -49,6 +55,8 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
 # HBM bytes of one step launch at each speculation depth, from the FETCH_SIZE /
 # WRITE_SIZE counter runs scaled by a plain copy (committed summary).
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_mcmc_step_registration256_spec.md")
+CHAIN_STEPS = 2000       # Metropolis steps of one registration chain
+DEPTHS = (16, 24, 32, 40, 48, 56, 64)  # speculation depths tuned over
 
 
 def measured_traffic(depth):
@@ -65,27 +73,85 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(problem, n_props, threads):
-    """The CPU oracle on this host: n_props proposals of the same chain step
-    (prep + calc_T_inv on the 65 536 pixel points + remap + SSD)."""
+def host_cpu():
+    """(logical CPUs this process may use, CPU model name) of this host."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return n, model
+
+
+def median_rate(fn, units, reps=5):
+    """One warm-up call, then the median over `reps` timed calls of fn():
+    (units per second, median seconds)."""
     import numpy as np
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    med = float(np.median(ts))
+    return units / med, med
+
+
+def cpu_baseline(problem, thetas, threads, n_single=4):
+    """The CPU oracle on this host, on proposals the GPU chain evaluated:
+    per proposal prep (sign, dt, expm) + calc_T_inv of the 65 536 pixel points
+    + bilinear remap + SSD.  Returns the rates on `threads` threads and on one
+    (the first n_single proposals), and the oracle's log-likelihoods."""
     from oracle import oracle
     oracle.lib()
     sp = problem.space
     g = sp.tessellation.geom()
     grid = sp.tessellation.dense_grid_img()
     p = problem.params
-    rng = np.random.default_rng(1)
-    thetas = [problem.theta_star + 0.01 * rng.standard_normal(len(problem.theta_star)) for _ in range(n_props)]
-    t0 = time.perf_counter()
-    for th in thetas:
-        Av = problem.B @ th
-        sA, T = oracle.prep(2, Av.reshape(sp.nC, -1), -1, p["dt"])
-        inv = oracle.calc_T(g, grid, T, sA, p["dt"], p["nTimeSteps"], p["nStepsODEsolver"], nthreads=threads)
-        w = oracle.remap2d(inv, problem.src, nthreads=threads)
-        oracle.ssd_ll(w, problem.dst, problem.sigma)
-    dt = time.perf_counter() - t0
-    return n_props / dt, dt
+    lls = []
+
+    def run(ths, nthreads, keep=False):
+        for th in ths:
+            Av = problem.B @ th
+            sA, T = oracle.prep(2, Av.reshape(sp.nC, -1), -1, p["dt"])
+            inv = oracle.calc_T(g, grid, T, sA, p["dt"], p["nTimeSteps"], p["nStepsODEsolver"], nthreads=nthreads)
+            w = oracle.remap2d(inv, problem.src, nthreads=nthreads)
+            ll = oracle.ssd_ll(w, problem.dst, problem.sigma)
+            if keep:
+                lls.append(ll)
+
+    run(thetas, threads, keep=True)  # the oracle's log-likelihoods (and the warm-up)
+    rate, med = median_rate(lambda: run(thetas, threads), len(thetas))
+    rate1, med1 = median_rate(lambda: run(thetas[:n_single], 1), n_single)
+    return rate, med, rate1, med1, lls
+
+
+def cpu_points_per_sec(threads):
+    """Config-1 unit of work on the CPU oracle: prep + calc_T_fwd of the
+    512 x 512 grid (262 144 points), on `threads` threads and on one."""
+    from cpabdiffeo_amd import configs
+    from oracle import oracle
+    cfg = configs.config1()
+    sp = cfg["space"]
+    g = sp.tessellation.geom()
+    As = cfg["Avees"].reshape(sp.nC, -1)
+
+    def once(n):
+        return lambda: oracle.transform(g, sp.nC, As, 1, cfg["pts"], cfg["dt"], cfg["nTimeSteps"], cfg["nStepsODE"],
+                                        nthreads=n)
+
+    npts = len(cfg["pts"])
+    rate, med = median_rate(once(threads), npts)
+    rate1, med1 = median_rate(once(1), npts)
+    return {"points_per_sec": rate, "threads": threads, "median_s": med,
+            "single_thread_points_per_sec": rate1, "single_thread_median_s": med1}
 
 
 def config1_extra(torch, ctx):
@@ -274,10 +340,10 @@ def config4_extra(torch, ctx, steps=200, lr=3e-4):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=10, help="timed chains (one bench step = one whole chain)")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed chains before the timed ones")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-proposals", type=int, default=64)
+    ap.add_argument("--cpu-proposals", type=int, default=32)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-seq", action="store_true", help="skip the one-step-per-launch comparison chain")
@@ -323,76 +389,113 @@ def main():
     problem = RegistrationProblem(n=n, seed=0)
     use_graph = not args.no_graph
     ctx = Context.get(0)
+    npts = n * n
+    per_prop = npts * (100 * F_FAST + F_REMAP_SSD)  # FLOPs of one proposal's pixels without fallbacks
 
-    def reset(mh):
-        for t in (mh.step_counter, mh.n_accepted, mh.n_fallback, mh.stats):
-            t.zero_()
+    def start(mh):
+        """theta = 0, step counter and acceptance count reset, ll(theta = 0) evaluated."""
+        mh.step_counter.zero_()
+        mh.n_accepted.zero_()
         mh.set_theta(np.zeros(mh.d))
+
+    def one_chain(mh):
+        start(mh)
+        mh.run(CHAIN_STEPS, use_graph=use_graph)
         torch.cuda.synchronize()
 
-    def chain(spec_depth, warmup, steps, timed_events, tune=()):
-        """warmup steps; with `tune`, the workload itself (steps from theta = 0)
-        at each candidate speculation depth, keeping the fastest; then the
-        timed chain: theta = 0, counters reset, `steps` steps."""
-        mh = DeviceMetropolis(problem, nChains=1, scale=0.01, seed=0, count_fallbacks=True, spec_depth=spec_depth)
-        mh.set_theta(np.zeros(mh.d))
-        mh.run(warmup, use_graph=use_graph)
-        torch.cuda.synchronize()
-        rates = {}
-        for K in tune:
-            try:
-                mh.set_spec_depth(K)
-            except ValueError:
-                continue
-            reset(mh)
+    def result(mh):
+        return (mh.theta.cpu().numpy()[0].copy(), float(mh.ll.cpu().numpy()[0]), int(mh.n_accepted.cpu()[0]))
+
+    # -- speculation depth: capture and warm every candidate on a whole chain,
+    # then time each on a whole chain (best of two) and keep the fastest
+    mh = DeviceMetropolis(problem, nChains=1, scale=0.01, seed=0, count_fallbacks=True, spec_depth=0)
+    start(mh)
+    log("bench: capture and warm-up of each speculation depth %s" % (DEPTHS,))
+    for K in DEPTHS:
+        mh.set_spec_depth(K)
+        one_chain(mh)
+    rates = {}
+    for K in DEPTHS:
+        mh.set_spec_depth(K)
+        best = None
+        for _ in range(2):
             t0 = time.perf_counter()
-            mh.run(steps, use_graph=use_graph)
-            torch.cuda.synchronize()
-            rates[K] = steps / (time.perf_counter() - t0)
-        if rates:
-            mh.set_spec_depth(max(rates, key=rates.get))
-        reset(mh)
-        if timed_events:
-            ctx.prof_enable(True)
-            ctx.prof_read()
-        t0 = time.perf_counter()
-        mh.run(steps, use_graph=use_graph)
-        torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
-        ev = ctx.prof_read() if timed_events else None
-        ctx.prof_enable(False)
-        out = dict(elapsed=elapsed, events=ev, stats=mh.stats.cpu().numpy()[0].copy(),
-                   nacc=int(mh.n_accepted.cpu().numpy()[0]), nfb=int(mh.n_fallback.cpu().numpy()[0]),
-                   ll=float(mh.ll.cpu().numpy()[0]), theta=mh.theta.cpu().numpy()[0].copy(), K=mh.spec_depth,
-                   tune=rates)
-        mh.close()
-        return out
+            one_chain(mh)
+            el = time.perf_counter() - t0
+            best = el if best is None else min(best, el)
+        rates[K] = CHAIN_STEPS / best
+    K_best = max(rates, key=rates.get)
+    mh.set_spec_depth(K_best)
+    log("bench: depth %d; %d warm-up chains, then %d timed chains of %d steps"
+        % (K_best, args.warmup, args.steps, CHAIN_STEPS))
+    for _ in range(args.warmup):
+        one_chain(mh)
 
-    log("bench: warmup %d steps, then the timed %d-step chain (speculative prefetch)" % (args.warmup, args.steps))
-    spec = chain(0, args.warmup, args.steps, True, tune=(16, 24, 32, 40, 48, 56, 64))
+    # -- timed region: args.steps whole chains
+    st0 = mh.stats.cpu().numpy()[0].copy()
+    nfb0 = int(mh.n_fallback.cpu()[0])
+    rep0, dir0 = mh.launch_counts()
+    ctx.prof_enable(True)
+    ctx.prof_read()
+    ends = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one_chain(mh)
+        ends.append(result(mh))
+    elapsed = time.perf_counter() - t0
+    cnt, ms = ctx.prof_read()
+    ctx.prof_enable(False)
+    st = mh.stats.cpu().numpy()[0] - st0
+    nfb = int(mh.n_fallback.cpu()[0]) - nfb0
+    rep1, dir1 = mh.launch_counts()
+    mh.close()
+    same_every_chain = all(np.array_equal(e[0], ends[0][0]) and e[1] == ends[0][1] and e[2] == ends[0][2]
+                           for e in ends)
+    chain_s = elapsed / args.steps
+    proposals_per_sec = CHAIN_STEPS / chain_s
+
+    # -- the same chain one step per launch; proposals it evaluates, for the CPU leg
     seq = None
+    samples = []
     if not args.no_seq:
         log("bench: the same chain one step per launch")
-        seq = chain(1, min(args.warmup, 200), args.steps, False)
-        same_chain = bool(np.array_equal(spec["theta"], seq["theta"]) and spec["ll"] == seq["ll"]
-                          and spec["nacc"] == seq["nacc"] and spec["nfb"] == seq["nfb"])
-    elapsed = spec["elapsed"]
-    proposals_per_sec = args.steps / elapsed
-    nacc = spec["nacc"]
+        m1 = DeviceMetropolis(problem, nChains=1, scale=0.01, seed=0, count_fallbacks=True, spec_depth=1)
+        one_chain(m1)  # capture, warm
+        t0 = time.perf_counter()
+        one_chain(m1)
+        el1 = time.perf_counter() - t0
+        r1 = result(m1)
+        seq = {"proposals_per_sec": CHAIN_STEPS / el1, "speedup_of_speculation": el1 / chain_s,
+               "same_chain_bit_for_bit": bool(np.array_equal(r1[0], ends[0][0]) and r1[1] == ends[0][1]
+                                              and r1[2] == ends[0][2])}
+        if not args.no_cpu:
+            # every (CHAIN_STEPS / cpu_proposals)-th proposal of the chain and the
+            # log-likelihood the GPU found for it
+            start(m1)
+            done = 0
+            stride = max(1, CHAIN_STEPS // args.cpu_proposals)
+            for s in range(0, CHAIN_STEPS, stride)[:args.cpu_proposals]:
+                if s > done:
+                    m1.run(s - done)
+                tp, _ = m1.last_proposal()
+                m1.run(1)
+                _, lp = m1.last_proposal()
+                done = s + 1
+                samples.append((s, tp.cpu().numpy()[0].copy(), float(lp.cpu().numpy()[0]) * npts))
+        m1.close()
 
-    # Roofline of the dominant kernel, live over the timed region: HIP events
-    # on the launch stream around every launch / graph replay, and the
-    # kernel's own counts of evaluated proposals and their fallback steps.
-    cnt, ms = spec["events"]
-    # stats: proposals evaluated in full, rejected Trel proposals of every pixel
-    # integrated, launches, steps decided, pixels integrated (slots above a
-    # known rejection are not started, so the work done is what was counted)
-    st = spec["stats"]
+    # -- roofline of the dominant kernel (k_mcmc_step + k_mcmc_tables), live over
+    # the timed region: HIP events on the launch stream around every graph replay
+    # and direct launch.  Algorithmic work = the decided proposals (2 000 steps and
+    # the evaluation of theta = 0 per chain) and their rejected-Trel redos; the
+    # kernel also reports what it integrated for speculative slots it discarded.
     launches = int(st[2])
     kern_s = ms / cnt * 1e-3
-    npts = n * n
-    flops_per_launch = (st[4] * (100 * F_FAST + F_REMAP_SSD) + st[1] * 10 * F_SUB) / launches
-    achieved = flops_per_launch / kern_s / 1e12
+    decided = int(st[3]) + args.steps
+    flops_dec = decided * per_prop + nfb * 10 * F_SUB
+    flops_int = st[4] * (100 * F_FAST + F_REMAP_SSD) + st[1] * 10 * F_SUB
+    achieved = flops_dec / launches / kern_s / 1e12
 
     extra = gd = batching = serve = None
     if not args.no_extra:
@@ -406,22 +509,28 @@ def main():
         serve = config5_extra(torch)
 
     cpu = None
-    if not args.no_cpu:
-        try:
-            threads = os.cpu_count() or 1
-            omp = os.environ.get("OMP_NUM_THREADS")
-            if omp:
-                threads = int(omp)
-            log("bench: CPU oracle baseline, %d proposals on %d threads" % (args.cpu_proposals, threads))
-            rate, secs = cpu_baseline(problem, args.cpu_proposals, threads)
-            rate1, secs1 = cpu_baseline(problem, 4, 1)
-            cpu = {"value": rate, "unit": "proposals/s", "cores": threads, "kind": "port",
-                   "sample": "%d proposals of this chain step (prep + calc_T_inv 65536 pts + remap + SSD), "
-                             "%.1f s wall on %d threads; single thread: %.3f proposals/s"
-                             % (args.cpu_proposals, secs, threads, rate1),
-                   "single_thread_value": rate1}
-        except Exception as e:  # the oracle is optional on a bare install
-            log("bench: CPU baseline unavailable: %r" % (e,))
+    if not args.no_cpu and samples:
+        threads, model = host_cpu()
+        omp = os.environ.get("OMP_NUM_THREADS")
+        if omp:
+            threads = min(threads, int(omp))
+        log("bench: CPU oracle baseline, %d proposals of the chain on %d threads (%s)" % (len(samples), threads, model))
+        thetas = [t for _, t, _ in samples]
+        rate, med, rate1, med1, lls = cpu_baseline(problem, thetas, threads)
+        rel = [abs(a - g) / abs(a) for a, (_, _, g) in zip(lls, samples)]
+        c1 = cpu_points_per_sec(threads)
+        cpu = {"value": rate, "unit": "proposals/s", "cores": threads, "kind": "port",
+               "sample": "%d proposals of the timed chain (every %dth step from 0), each: prep + calc_T_inv of "
+                         "65536 pts + remap + SSD; 1 warm-up pass, median of 5 passes (%.2f s) on %d threads; "
+                         "single thread: first 4 proposals, %.3f proposals/s (median %.2f s)"
+                         % (len(samples), max(1, CHAIN_STEPS // args.cpu_proposals), med, threads, rate1, med1),
+               "single_thread_value": rate1, "cpu_model": model, "nproc": host_cpu()[0],
+               "build": "gcc -O3 -fopenmp -ffp-contract=off, static schedule over points",
+               "ll_max_rel_err_gpu_vs_oracle": max(rel),
+               "calc_T_fwd_512x512_points_per_sec": c1}
+        if extra is not None:
+            extra["cpu_points_per_sec"] = c1["points_per_sec"]
+            extra["gpu_vs_cpu"] = extra["points_per_sec"] / c1["points_per_sec"]
 
     line = {
         "metric": METRIC,
@@ -430,32 +539,34 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": chain_s * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "mcmc-registration 256x256 fp32 blob image, 1 chain from theta=0, %d steps, "
-                               "nC=64 tess I, calc_T_inv t=1 100x10 fp64 + bilinear remap + SSD sigma=0.02"
-                               % args.steps,
-                   "points_per_proposal": npts, "graph_replay": use_graph, "spec_depth": spec["K"],
-                   "spec_depth_autotune_proposals_per_sec": {str(k): v for k, v in spec["tune"].items()},
-                   "accept_ratio": nacc / float(args.steps)},
+        "config": {"workload": "mcmc-registration 256x256 fp32 blob image, one chain of %d Metropolis steps from "
+                               "theta=0 per bench step, nC=64 tess I, calc_T_inv t=1 100x10 fp64 + bilinear remap + "
+                               "SSD sigma=0.02" % CHAIN_STEPS,
+                   "chain_steps": CHAIN_STEPS, "points_per_proposal": npts, "spec_depth": K_best,
+                   "spec_depth_autotune_proposals_per_sec": {str(k): v for k, v in rates.items()},
+                   "graph_replayed_launches": rep1 - rep0, "direct_launches": dir1 - dir0,
+                   "accept_ratio": ends[0][2] / float(CHAIN_STEPS),
+                   "every_timed_chain_identical": bool(same_every_chain)},
         "points_per_sec": proposals_per_sec * npts,
-        "one_step_per_launch": None if seq is None else {
-            "proposals_per_sec": args.steps / seq["elapsed"], "speedup_of_speculation": seq["elapsed"] / elapsed,
-            "same_chain_bit_for_bit": same_chain},
+        "one_step_per_launch": seq,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(spec["K"]),
-                     "kernel": "k_mcmc_step + k_mcmc_tables (K proposals: calc_T_inv + remap + SSD; accept; "
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(K_best),
+                     "kernel": "k_mcmc_step + k_mcmc_tables (S proposals: calc_T_inv + remap + SSD; accept; "
                                "arm next)",
-                     "kernel_us": kern_s * 1e6, "launches": launches,
-                     "proposals_evaluated_per_launch": st[0] / launches,
+                     "kernel_us": kern_s * 1e6, "launches": launches, "event_launches": int(cnt),
+                     "decided_proposals_per_launch": decided / launches,
+                     "alg_flops_decided_per_launch": flops_dec / launches,
+                     "fallback_steps_decided_per_launch": nfb / launches,
                      "proposals_integrated_per_launch": st[4] / float(npts) / launches,
-                     "steps_decided_per_launch": st[3] / launches,
-                     "alg_flops_per_launch": flops_per_launch,
-                     "fallback_steps_per_launch": st[1] / launches},
+                     "alg_flops_integrated_per_launch": flops_int / launches,
+                     "frac_counting_discarded_slots": flops_int / launches / kern_s / 1e12 / FP64_PEAK_TFLOPS,
+                     "end_to_end_frac": flops_dec / elapsed / 1e12 / FP64_PEAK_TFLOPS},
         "cpu_baseline": cpu,
         "calc_T_fwd_512x512": extra,
         "gradient_descent_64x128x128": gd,
