@@ -51,6 +51,7 @@ SIGNATURES = {
     "cpab_reg2d_last_proposal": [_p, _p, _p],
     "cpab_reg2d_spec_depth": [_p, ctypes.POINTER(_i)],
     "cpab_reg2d_set_spec_depth": [_p, _i],
+    "cpab_reg2d_launch_counts": [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
     "cpab_reg2d_destroy": [_p],
     "cpab_prof_enable": [_p, _i],
     "cpab_prof_read": [_p, ctypes.POINTER(_i64), ctypes.POINTER(_d)],

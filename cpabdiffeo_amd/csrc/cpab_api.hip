@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -1079,9 +1080,11 @@ struct cpab_reg2d {
   int block = MCMC_BLOCK;  // threads per block of the step kernel
   bool pow2 = true;
   double* scratch = nullptr;  // theta_prop | ll_prop | tables | partials | slot_res | lz | Bt | done_slot | stop | done
-  hipStream_t cap = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;  // capture stream, created once
+  // one captured kGraphSteps-launch graph per speculation depth: the launches
+  // carry the descriptor by value and only D.K differs between depths
+  std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+  int64_t n_replayed = 0, n_direct = 0;  // step launches by graph replay / directly
   std::vector<int64_t> host_step;
 };
 
@@ -1162,33 +1165,46 @@ int reg2d_launches(cpab_reg2d* r, int64_t n, int use_graph) {
     for (int64_t i = 0; i < n; i++) {
       int rc = reg2d_launch_step(r, ctx->stream, 0);
       if (rc) return rc;
+      r->n_direct++;
     }
     return CPAB_OK;
   }
-  if (!r->exec) {
+  const int64_t full = n / kGraphSteps;
+  hipGraphExec_t exec = nullptr;
+  auto it = r->graphs.find(r->D.K);
+  if (it != r->graphs.end()) exec = it->second.second;
+  if (!exec && full > 0) {
     // capture kGraphSteps launches on a private stream; replays go to the context's stream
-    HIP_OK(ctx, hipStreamCreate(&r->cap));
+    if (!r->cap) HIP_OK(ctx, hipStreamCreate(&r->cap));
     HIP_OK(ctx, hipStreamBeginCapture(r->cap, hipStreamCaptureModeThreadLocal));
     int rc = CPAB_OK;
     for (int i = 0; i < kGraphSteps && rc == CPAB_OK; i++) rc = reg2d_launch_step(r, r->cap, 0);
     hipGraph_t gph = nullptr;
     hipError_t e = hipStreamEndCapture(r->cap, &gph);
-    if (rc) return rc;
+    if (rc) {
+      if (gph) (void)hipGraphDestroy(gph);
+      return rc;
+    }
     if (e != hipSuccess) return fail(ctx, CPAB_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    r->graph = gph;
-    HIP_OK(ctx, hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0));
+    e = hipGraphInstantiate(&exec, gph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(gph);
+      return fail(ctx, CPAB_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    r->graphs[r->D.K] = std::make_pair(gph, exec);
   }
-  const int64_t full = n / kGraphSteps;
   for (int64_t i = 0; i < full; i++) {
     hipEvent_t a = nullptr, b = nullptr;
     int rc = prof_begin(ctx, &a, &b);
     if (rc) return rc;
-    HIP_OK(ctx, hipGraphLaunch(r->exec, ctx->stream));
+    HIP_OK(ctx, hipGraphLaunch(exec, ctx->stream));
     if ((rc = prof_end(ctx, a, b, kGraphSteps))) return rc;
+    r->n_replayed += kGraphSteps;
   }
   for (int64_t i = full * kGraphSteps; i < n; i++) {
     int rc = reg2d_launch_step(r, ctx->stream, 0);
     if (rc) return rc;
+    r->n_direct++;
   }
   return CPAB_OK;
 }
@@ -1300,12 +1316,12 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   }
   r->K_cap = K_cap;
   // scratch, zeroed once: theta_prop | ll_prop | tables | partial | partial_nfb | partial_px | slot_res |
-  // lz | Bt | done_slot | stop | done | order | ready, abort_after (in doubles; done_slot starts on a
-  // 128-byte boundary)
+  // lz | Bt | lp | lp_prop | done_slot | stop | done | order | ready, abort_after (in doubles; done_slot
+  // starts on a 128-byte boundary)
   const size_t nck = (size_t)q.nChains * K_cap;
   const size_t nB = (size_t)q.nC * 6 * q.d;
-  const size_t o_ds =
-      (nck * q.d + q.nChains + nck * 2 * q.nC * 6 + 3 * nck * D.nBlocks + 3 * nck + 2 * nck * q.d + nB + 15) / 16 * 16;
+  const size_t o_ds = (nck * q.d + q.nChains + nck * 2 * q.nC * 6 + 3 * nck * D.nBlocks + 3 * nck + 2 * nck * q.d +
+                       nB + q.nChains + nck + 15) / 16 * 16;
   const size_t n_ds = nck * DONE_STRIDE / 2;
   const size_t n_ord = ((size_t)q.nChains * D.nBlocks + 1) / 2;  // ints, in doubles
   const size_t o_ctl = o_ds + n_ds + 2 * q.nChains + n_ord;
@@ -1328,6 +1344,10 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   D.lz = D.slot_res + 3 * nck;
   double* Bt = D.lz + 2 * nck * q.d;
   D.Bt = Bt;
+  D.lp = Bt + nB;
+  D.lp_prop = D.lp + q.nChains;
+  D.wlp = q.wlp;
+  D.J = q.prior_prec;
   D.done_slot = reinterpret_cast<unsigned int*>(r->scratch + o_ds);
   D.stop = reinterpret_cast<int64_t*>(r->scratch + o_ds + n_ds);
   D.done = reinterpret_cast<unsigned int*>(D.stop + q.nChains);
@@ -1425,12 +1445,9 @@ int cpab_reg2d_set_spec_depth(cpab_reg2d* r, int K) {
   if (K < 1 || K > r->K_cap) return fail(ctx, CPAB_ERR_ARG, "spec_depth must be in [1, %d] for this chain set", r->K_cap);
   if (K == D.K) return CPAB_OK;
   if ((int64_t)D.nBlocks * K > 0x7fffffff) return fail(ctx, CPAB_ERR_ARG, "image too large for the launch grid");
-  // the captured launches carry the old descriptor
+  // launches in flight read the scratch the re-arming below rewrites; graphs
+  // captured at other depths stay (the descriptor differs only in K)
   HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
-  if (r->exec) (void)hipGraphExecDestroy(r->exec);
-  if (r->graph) (void)hipGraphDestroy(r->graph);
-  r->exec = nullptr;
-  r->graph = nullptr;
   D.K = K;
   // re-arm the proposals of the current step at the new depth: the same
   // numbers the chain would use, so the chain is unchanged
@@ -1441,11 +1458,20 @@ int cpab_reg2d_set_spec_depth(cpab_reg2d* r, int K) {
   return CPAB_OK;
 }
 
+int cpab_reg2d_launch_counts(cpab_reg2d* r, int64_t* replayed, int64_t* direct) {
+  if (!r) return CPAB_ERR_ARG;
+  if (replayed) *replayed = r->n_replayed;
+  if (direct) *direct = r->n_direct;
+  return CPAB_OK;
+}
+
 int cpab_reg2d_destroy(cpab_reg2d* r) {
   if (!r) return CPAB_OK;
   (void)hipStreamSynchronize(r->ctx->stream);
-  if (r->exec) (void)hipGraphExecDestroy(r->exec);
-  if (r->graph) (void)hipGraphDestroy(r->graph);
+  for (auto& g : r->graphs) {
+    (void)hipGraphExecDestroy(g.second.second);
+    (void)hipGraphDestroy(g.second.first);
+  }
   if (r->cap) (void)hipStreamDestroy(r->cap);
   if (r->scratch) (void)hipFree(r->scratch);
   delete r;

@@ -14,7 +14,8 @@
 //             slot (acquire) sums them in a fixed order: ll' = -0.5/sigma^2
 //             sum (divided by the pixel count when use_ave_ll,
 //             Metropolis.py:58-59); the last block of the chain (acquire)
-//             forms log_ratio = ll' - ll, accepts when log_ratio > 0 or
+//             forms log_ratio = ll' - ll (+ wlp (lp' - lp) with a prior,
+//             Metropolis.py:64-70), accepts when log_ratio > 0 or
 //             log(u) < log_ratio (with `inverted` the reference's test
 //             log(u) > log_ratio, Metropolis.py:77-82), slot after slot until
 //             the first rejection, advances the step counter by the slots
@@ -129,18 +130,22 @@ struct McmcDesc {
   int poll;             // abort checkpoints per pixel run (one pixel per lane only; 0: none)
   int* order;           // (nChains, nBlocks): the pixel block each dispatch position integrates,
                         // or null: dispatch position = pixel block
+  double wlp;           // weight of the log-prior ratio
+  const double* J;      // (d, d) row-major prior precision, or null: no prior term
+  double* lp;           // (nChains,) log-prior of the current theta (with J)
+  double* lp_prop;      // (nChains, K) log-prior of each armed proposal (with J)
   Geom g;
 };
 
 // LDS of the last block's epilogue (doubles): ll', fallback count,
-// acceptance uniform and pixel count per slot.
-__host__ __device__ inline int mcmc_finish_doubles(int K) { return 4 * K; }
+// acceptance uniform, pixel count and log-prior per slot.
+__host__ __device__ inline int mcmc_finish_doubles(int K) { return 5 * K; }
 // LDS of k_mcmc_prep (doubles): normals and proposals of every slot.
 __host__ __device__ inline int mcmc_arm_doubles(int K, int d) { return 2 * K * d; }
-// LDS of k_mcmc_tables (doubles): one proposal and its Avees, two normals, or
-// the block counts of the dispatch ordering.
+// LDS of k_mcmc_tables (doubles): one proposal, its Avees and its log-prior
+// terms, two normals, or the block counts of the dispatch ordering.
 __host__ __device__ inline int mcmc_tables_doubles(int nC, int d, int nBlocks) {
-  int n = d + 6 * nC > 2 * d ? d + 6 * nC : 2 * d;
+  int n = 2 * d + 6 * nC;
   const int o = nBlocks < 1024 ? nBlocks : 1024;
   return n > o ? n : o;
 }
@@ -305,6 +310,23 @@ __global__ void __launch_bounds__(1024) k_mcmc_tables(McmcDesc D, int nk) {
     a[r] = v;
     T[R + r] = v;
   }
+  if (D.J) {
+    // lp' = -0.5 (theta'^T J) theta' (ScaleDependentLogPrior.calc_lp, mu = 0):
+    // (theta'^T J)_j summed over i in index order, then the products summed
+    // in index order, so a proposal's lp' does not depend on its slot
+    double* pr = a + R;
+    for (int j = threadIdx.x; j < d; j += blockDim.x) {
+      double v = 0.0;
+      for (int i = 0; i < d; i++) v += th[i] * D.J[(int64_t)i * d + j];
+      pr[j] = v * th[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double q = 0.0;
+      for (int j = 0; j < d; j++) q += pr[j];
+      D.lp_prop[(int64_t)ch * D.K + m] = -0.5 * q;
+    }
+  }
   __syncthreads();
   for (int c = threadIdx.x; c < nC; c += blockDim.x) {
     double adt[6], t6[6];
@@ -349,11 +371,13 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
   double* nfs = w + K;
   double* lus = nfs + K;
   double* pxs = lus + K;
+  double* lpp = pxs + K;
   for (int m = threadIdx.x; m < kact; m += blockDim.x) {
     const double* r = D.slot_res + ((int64_t)ch * K + m) * 3;
     llp[m] = __hip_atomic_load(r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     nfs[m] = __hip_atomic_load(r + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     pxs[m] = __hip_atomic_load(r + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (D.J) lpp[m] = D.lp_prop[(int64_t)ch * K + m];  // written by the tables launch before this one
     // log of the slot's acceptance uniform (read only if its ll does not improve)
     if (!init) {
       const int64_t s = step + m;
@@ -365,6 +389,7 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
   __syncthreads();
   if (threadIdx.x == 0) {
     double ll = D.ll[ch];
+    double lp = D.J ? D.lp[ch] : 0.0;
     int nd = 0, la = -1;
     int64_t nacc = 0;
     double nf_dec = 0.0, nf_all = 0.0, px_all = 0.0;
@@ -377,7 +402,8 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
       if (init) {
         acc = true;
       } else {
-        const double lr = llp[m] - ll;
+        double lr = llp[m] - ll;
+        if (D.J) lr += D.wlp * (lpp[m] - lp);  // Metropolis.py:64-70
         if (lr > 0) {
           acc = true;
         } else {
@@ -389,6 +415,7 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
       nf_dec += nfs[m];
       if (acc) {
         ll = llp[m];
+        if (D.J) lp = lpp[m];
         la = m;
         nacc++;
       }
@@ -397,6 +424,7 @@ __device__ inline void mcmc_finish(const McmcDesc& D, int ch, int64_t step, int 
     }
     D.ll_prop[ch] = llp[nd - 1];
     D.ll[ch] = ll;
+    if (D.J) D.lp[ch] = lp;
     if (!init) {
       D.n_acc[ch] += nacc;
       D.step[ch] = step + nd;

@@ -38,7 +38,8 @@ class RegDesc(ctypes.Structure):
                 ("src", ctypes.c_void_p), ("dst", ctypes.c_void_p), ("theta", ctypes.c_void_p),
                 ("ll", ctypes.c_void_p), ("n_accepted", ctypes.c_void_p), ("step", ctypes.c_void_p),
                 ("trace", ctypes.c_void_p), ("trace_len", ctypes.c_int), ("n_fallback", ctypes.c_void_p),
-                ("spec_depth", ctypes.c_int), ("stats", ctypes.c_void_p), ("pts_per_wave", ctypes.c_int)]
+                ("spec_depth", ctypes.c_int), ("stats", ctypes.c_void_p), ("pts_per_wave", ctypes.c_int),
+                ("wlp", ctypes.c_double), ("prior_prec", ctypes.c_void_p)]
 
 
 def pixel_basis(space_unit, n):
@@ -89,11 +90,15 @@ class DeviceMetropolis:
     (1: one step per launch; K > 1: speculative prefetch of the next K steps,
     the same chain bit for bit; 0: chosen by the library).  pts_per_wave:
     pixels per wave (0: by the launch size); chains run with the same value
-    give the same bits alone and in a batch."""
+    give the same bits alone and in a batch.  use_prior: add the log-prior
+    ratio wlp (lp' - lp), lp = -0.5 theta^T Sigma_CPA^-1 theta, to the
+    acceptance test as Register does (Metropolis.py:64-70,
+    Register.py:224-248, wlp = 1e-4 there); prior_prec overrides the
+    precision matrix."""
 
     def __init__(self, problem, nChains=1, scale=0.01, diagonal=False, use_ave_ll=True,
                  inverted_acceptance=False, seed=0, trace_len=0, count_fallbacks=False, spec_depth=1,
-                 pts_per_wave=0):
+                 pts_per_wave=0, use_prior=False, wlp=1e-4, prior_prec=None):
         torch = require_gpu()
         self.problem = problem
         self.nChains = nChains
@@ -123,6 +128,12 @@ class DeviceMetropolis:
         # proposals evaluated in full, rejected Trel proposals of every pixel integrated,
         # launches, steps decided, pixels integrated
         self.stats = torch.zeros((nChains, 5), dtype=torch.int64, device=dev)
+        self.wlp = float(wlp) if use_prior else 0.0
+        self.prior_prec = None
+        if use_prior:
+            J = np.linalg.inv(cov) if prior_prec is None else np.asarray(prior_prec, dtype=np.float64)
+            self.prior_prec = J
+            self._J = torch.from_numpy(np.ascontiguousarray(J)).to(dev)
         p = problem.params
         T = sp.tessellation
         desc = RegDesc(nChains, d, T.type, sp.nC, int(T.nCs[0]), int(T.nCs[1]), float(T.incs[0]),
@@ -132,7 +143,8 @@ class DeviceMetropolis:
                        self._dst.data_ptr(), self.theta.data_ptr(), self.ll.data_ptr(), self.n_accepted.data_ptr(),
                        self.step_counter.data_ptr(), self.trace.data_ptr() if trace_len else None, int(trace_len),
                        self.n_fallback.data_ptr() if count_fallbacks else None, int(spec_depth),
-                       self.stats.data_ptr(), int(pts_per_wave))
+                       self.stats.data_ptr(), int(pts_per_wave), self.wlp,
+                       self._J.data_ptr() if use_prior else None)
         self._ctx = Context.get(0)
         self._ctx.bind_stream(torch.cuda.current_stream().cuda_stream)
         h = ctypes.c_void_p()
@@ -175,6 +187,12 @@ class DeviceMetropolis:
         check(lib().cpab_reg2d_set_spec_depth(self._h, int(K)), self._ctx.handle)
         self.spec_depth = int(K)
 
+    def launch_counts(self):
+        """(step launches replayed from captured graphs, launched directly) so far."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().cpab_reg2d_launch_counts(self._h, ctypes.byref(a), ctypes.byref(b)), self._ctx.handle)
+        return a.value, b.value
+
     def autotune_spec(self, candidates=(8, 16, 24, 32, 40, 48, 56, 64), steps=192, use_graph=True):
         """Time `steps` Metropolis steps of these chains at each candidate
         speculation depth (the steps are real steps of the chains) and keep
@@ -186,7 +204,7 @@ class DeviceMetropolis:
                 self.set_spec_depth(K)
             except ValueError:
                 continue  # beyond what this chain set was sized for
-            self.run(16, use_graph=use_graph)  # capture and warm at this depth
+            self.run(16 * K, use_graph=use_graph)  # capture (a graph is kept per depth) and warm
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             self.run(steps, use_graph=use_graph)

@@ -239,8 +239,9 @@ int cpab_calc_T_items(cpab_ctx *ctx, int dim, int tess, int nItems, const cpab_w
  *            counter (step, chain), key = seed);
  *   Avees  = B theta';  T = calc_T_inv(grid; Avees);
  *   ll'    = sum -0.5 (src(T(x)) - dst(x))^2 / sigma^2  (/ H*W if use_ave_ll);
- *   accept if ll' > ll, else if log(u) < ll' - ll (log(u) > ll' - ll with
- *   inverted_acceptance, the reference's test, Metropolis.py:77-82).
+ *   log_ratio = ll' - ll (+ wlp (lp' - lp) with a prior, Metropolis.py:64-70);
+ *   accept if log_ratio > 0, else if log(u) < log_ratio (log(u) > log_ratio
+ *   with inverted_acceptance, the reference's test, Metropolis.py:77-82).
  * All arrays are device pointers owned by the caller; the object owns only
  * its scratch.  B is copied (transposed) when the object is created, on the
  * context's stream; later changes to B are not seen. ----------------------- */
@@ -292,6 +293,13 @@ typedef struct {
                            assignment to blocks, so chains give the same bits alone
                            and in a batch when they run with the same explicit value
                            (4-wave blocks then). */
+  double wlp;           /* weight of the log-prior ratio (Metropolis wlp, :15, 64-70) */
+  const double *prior_prec; /* optional (d, d) row-major precision J of the Gaussian prior
+                           N(0, J^-1) on theta (ScaleDependentLogPrior.calc_lp,
+                           cpab/cpaNd/model/_ScaleDependentLogPrior.py: lp = -0.5 theta^T J
+                           theta, mu = 0); when set, log_ratio = ll' - ll + wlp (lp' - lp)
+                           (Metropolis.py:64-70; Register.py:224-248 passes one with
+                           wlp = 1e-4).  NULL: no prior term.  Read at every launch. */
 } cpab_reg2d_desc;
 
 typedef struct cpab_reg2d cpab_reg2d;
@@ -315,8 +323,13 @@ int cpab_reg2d_spec_depth(cpab_reg2d *r, int *K);
 /* Change the speculation depth between runs (up to four times the library's
  * own choice at create, at most 64, or the depth asked for if larger); the armed proposals
  * are recomputed for the new depth, so the chain is unchanged.  Call after
- * cpab_reg2d_init.  Synchronises the context's stream. */
+ * cpab_reg2d_init.  Synchronises the context's stream.  The graph captured at
+ * each depth is kept, so returning to a depth replays it without capturing. */
 int cpab_reg2d_set_spec_depth(cpab_reg2d *r, int K);
+/* Step launches so far: by replay of a captured graph and launched directly
+ * (no reference counterpart; the reference launches every kernel of a step
+ * from the host, Metropolis.py:41-102).  Either pointer may be NULL. */
+int cpab_reg2d_launch_counts(cpab_reg2d *r, int64_t *replayed, int64_t *direct);
 int cpab_reg2d_destroy(cpab_reg2d *r);
 
 /* Kernel timing with HIP events on the context's stream (no reference

@@ -61,17 +61,37 @@ def log_likelihood(orc, space, B, theta, src, dst, sigma, params, use_ave_ll=Tru
     return ll / (n * n) if use_ave_ll else ll
 
 
+def log_prior(J, theta):
+    """lp = -0.5 theta^T J theta (cpab/cpaNd/model/_ScaleDependentLogPrior.py,
+    mu = 0): (theta^T J)_j summed over i in index order, then the products
+    theta_j (theta^T J)_j summed in index order, in IEEE double."""
+    d = len(theta)
+    q = 0.0
+    for j in range(d):
+        v = 0.0
+        for i in range(d):
+            v += float(theta[i]) * float(J[i, j])
+        q += v * float(theta[j])
+    return -0.5 * q
+
+
 def run_chain(orc, space, B, L, theta0, src, dst, sigma, params, nsteps, seed, chain=0,
-              use_ave_ll=True, inverted=False):
-    """Returns per-step (accepted, theta after the step, ll after the step, proposal ll)."""
+              use_ave_ll=True, inverted=False, J=None, wlp=0.0):
+    """Returns per-step (accepted, theta after the step, ll after the step, proposal ll).
+    With a prior precision J the log ratio gains wlp (lp' - lp)
+    (cpab/cpaNd/inference/Metropolis.py:64-70)."""
     theta = np.array(theta0, dtype=np.float64)
     ll = log_likelihood(orc, space, B, theta, src, dst, sigma, params, use_ave_ll)
+    lp = log_prior(J, theta) if J is not None else 0.0
     out = []
     for step in range(nsteps):
         z = normals(step, chain, len(theta), seed)
         prop = theta + np.array([sum(L[i, j] * z[j] for j in range(i + 1)) for i in range(len(theta))])
         llp = log_likelihood(orc, space, B, prop, src, dst, sigma, params, use_ave_ll)
         lr = llp - ll
+        if J is not None:
+            lpp = log_prior(J, prop)
+            lr += wlp * (lpp - lp)
         if lr > 0:
             acc = True
         else:
@@ -79,5 +99,7 @@ def run_chain(orc, space, B, L, theta0, src, dst, sigma, params, nsteps, seed, c
             acc = (lu > lr) if inverted else (lu < lr)
         if acc:
             theta, ll = prop, llp
+            if J is not None:
+                lp = lpp
         out.append((acc, theta.copy(), ll, llp))
     return out

@@ -67,6 +67,39 @@ def test_chain_matches_cpu_reference(problem, problem60, orc, gpu, spec, size, p
     assert 0 < nacc.sum() < 2 * nsteps  # both branches of the test ran
 
 
+@pytest.mark.parametrize("spec", [1, 4])
+def test_chain_with_log_prior_matches_cpu_reference(problem, orc, gpu, spec):
+    """The log-prior term of Metropolis.step (Metropolis.py:64-70:
+    log_ratio += wlp (lp' - lp), lp = -0.5 theta^T Sigma_CPA^-1 theta as
+    ScaleDependentLogPrior computes it): the device chain with the prior
+    equals the CPU reference chain with it, and the term changes the chain
+    (a weight large enough to matter at this proposal scale)."""
+    from cpabdiffeo_amd.registration import DeviceMetropolis
+    say("chain with log-prior vs CPU reference, spec_depth", spec)
+    p = problem
+    nsteps, wlp = 16, 2.0
+    runs = {}
+    for use_prior in (True, False):
+        mh = DeviceMetropolis(p, nChains=2, scale=0.05, seed=21, trace_len=nsteps, spec_depth=spec,
+                              use_prior=use_prior, wlp=wlp)
+        mh.set_theta(np.zeros(mh.d))
+        mh.run(nsteps, use_graph=False)
+        runs[use_prior] = (mh.trace.cpu().numpy(), mh.theta.cpu().numpy(), mh.n_accepted.cpu().numpy(),
+                           mh.prior_prec, mh.Lprop)
+        mh.close()
+    trace, theta, nacc, J, Lp = runs[True]
+    for ch in range(2):
+        ref = run_chain(orc, p.space, p.B, Lp, np.zeros(len(theta[ch])), p.src, p.dst, p.sigma, p.params, nsteps,
+                        seed=21, chain=ch, J=J, wlp=wlp)
+        acc = [r[0] for r in ref]
+        assert nacc[ch] == sum(acc), (ch, nacc[ch], acc)
+        np.testing.assert_allclose(trace[ch], [r[2] for r in ref], rtol=1e-9)
+        np.testing.assert_allclose(theta[ch], ref[-1][1], rtol=1e-12, atol=1e-14)
+    assert 0 < nacc.sum() < 2 * nsteps
+    # the prior term is active: the chains differ from the ones without it
+    assert not np.array_equal(runs[False][0], trace)
+
+
 def test_graph_replay_equals_launches_and_batch_equals_alone(problem, gpu):
     import torch
     from cpabdiffeo_amd.registration import DeviceMetropolis
