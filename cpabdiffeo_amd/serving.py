@@ -82,6 +82,7 @@ class Request:
         self.theta = theta
         self.t_admit = self.t_done = None
         self.result = None
+        self.error = None  # set when the request is refused (larger than the whole budget)
 
 
 def make_mix(n_req, kinds, rate=2000.0, seed=0, n1d=(1e4, 1e6), sides=(64, 128, 256, 512, 1024)):
@@ -244,7 +245,10 @@ class Server:
     def run(self, requests, keep=False, realtime=True):
         """Serve `requests` (sorted by arrival).  Returns a stats dict; with
         keep, every request keeps its input points and result in .inputs and
-        .result."""
+        .result, and its tables (Trels, signed As) and transformed points in
+        .tables and .points_out.  A request that needs more device memory than
+        the whole budget can never be admitted: it is refused when it arrives
+        (its .error says why, the stats count it) and the others are served."""
         torch = self.torch
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
@@ -252,13 +256,20 @@ class Server:
         flights = collections.deque()
         used = peak = 0
         iters = shared = 0
-        done = []
+        done, refused = [], []
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
         while pending or waiting or active or flights:
             t = now()
             while pending and (pending[0].arrival <= t or not realtime):
-                waiting.append(pending.popleft())
+                r = pending.popleft()
+                need = self.bytes_of(r)
+                if need > self.budget:
+                    r.error = ValueError("request %d (%s, %d points) needs %d device bytes, more than the whole "
+                                         "budget of %d" % (r.id, r.kind, r.n, need, self.budget))
+                    refused.append(r)
+                    continue
+                waiting.append(r)
             new = []
             while waiting and used + self.bytes_of(waiting[0]) <= self.budget:
                 r = waiting.popleft()
@@ -292,6 +303,8 @@ class Server:
                         if keep:
                             r.inputs = r.pts
                             r.result = r.warped if r.kind == "3d" else r.out
+                            r.tables = (r.trels, r.As)
+                            r.points_out = r.out
                         r.pts = r.out = r.trels = r.As = r.warped = None
                         done.append(r)
                 active = [r for r in active if r.left > 0]
@@ -300,17 +313,17 @@ class Server:
                     time.sleep(2e-5)
                 elif pending and realtime:
                     time.sleep(max(0.0, min(pending[0].arrival - now(), 1e-3)))
-        makespan = now()
+        makespan = max(now(), 1e-9)
         lat = np.array([r.t_done - r.arrival for r in done])
+        pct = lambda a, q: float(np.percentile(a, q) * 1e3) if len(a) else None
         pts = sum(r.n for r in done)
         per_kind = {}
         for k in sorted(set(r.kind for r in done)):
             lk = np.array([r.t_done - r.arrival for r in done if r.kind == k])
-            per_kind[k] = {"n": int(len(lk)), "p50_ms": float(np.percentile(lk, 50) * 1e3),
-                           "p99_ms": float(np.percentile(lk, 99) * 1e3)}
-        return {"requests": len(done), "makespan_s": makespan, "requests_per_sec": len(done) / makespan,
-                "points_per_sec": pts / makespan, "p50_ms": float(np.percentile(lat, 50) * 1e3),
-                "p99_ms": float(np.percentile(lat, 99) * 1e3), "peak_bytes": int(peak), "iterations": iters,
+            per_kind[k] = {"n": int(len(lk)), "p50_ms": pct(lk, 50), "p99_ms": pct(lk, 99)}
+        return {"requests": len(done), "refused": len(refused), "makespan_s": makespan,
+                "requests_per_sec": len(done) / makespan, "points_per_sec": pts / makespan, "p50_ms": pct(lat, 50),
+                "p99_ms": pct(lat, 99), "peak_bytes": int(peak), "iterations": iters,
                 "iterations_with_several_requests": shared, "per_kind": per_kind}
 
     def autotune(self, requests, candidates=((1024, 1024, 1 << 21), (2048, 1024, 1 << 21), (4096, 1024, 1 << 21),
