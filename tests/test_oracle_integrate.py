@@ -155,3 +155,43 @@ def test_sensitivity_matches_finite_differences(orc):
     err = np.abs(fd - grad).max(axis=(1, 2))
     # a point whose cell sequence changes under +-h is not differentiable there
     assert (err < 1e-6).mean() > 0.9, np.sort(err)[-5:]
+
+
+def _bary_velocity(sp, Avees, X):
+    """v(X) = A_c [X; 1] with the simplex c found by barycentric coordinates
+    (largest smallest coordinate), any dimension: independent of the
+    reference's fmod-based lookup and of the oracle's affine product."""
+    dim = sp.dim_domain
+    V = sp.tessellation.cells_verts_homo_coo  # (nC, dim + 1, dim + 1)
+    Minv = np.linalg.inv(np.transpose(V, (0, 2, 1)))
+    Xh = np.concatenate([X, np.ones((len(X), 1))], 1)
+    lam = np.einsum("cij,pj->pci", Minv, Xh)
+    c = np.argmax(lam.min(axis=2), axis=1)
+    A = Avees.reshape(sp.nC, dim, dim + 1)
+    return np.einsum("pij,pj->pi", A[c], Xh), lam.min(axis=2).max(axis=1)
+
+
+@pytest.mark.parametrize("nCs", [(4, 4), (3, 5), (2, 2, 2)])
+def test_calc_v_matches_barycentric_cell_velocity(orc, nCs):
+    """oracle calc_v (cpab/gpu/calc_2Dto2D.cu:561-602, calc_3Dto3D.cu) against
+    the velocity of the simplex that contains each point, found by
+    barycentric coordinates; a wrong cell, a transposed table or a dropped
+    translation term would differ (the field is continuous, but the affine
+    extension of a neighbouring cell is not the cell's own field)."""
+    dim = len(nCs)
+    sp = CpaSpace([0] * dim, [1] * dim, list(nCs), zero_v_across_bdry=[True] * dim, tess="I",
+                  valid_outside=False)
+    rng = np.random.default_rng(31 + dim)
+    Av = sp.B @ rng.normal(size=sp.d)
+    X = rng.uniform(0.0, 1.0, (4000, dim))
+    want, depth = _bary_velocity(sp, Av, X)
+    got = orc.calc_v(sp.tessellation.geom(), X, Av.reshape(sp.nC, -1))
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(Av).max())
+    # the test can see a wrong cell: for interior points another cell's affine
+    # field evaluated at the point differs clearly
+    inside = depth > 1e-3
+    A = Av.reshape(sp.nC, dim, dim + 1)
+    Xh = np.concatenate([X, np.ones((len(X), 1))], 1)
+    c_idx = orc.calc_cell_idx(sp.tessellation.geom(), X)
+    wrong = np.einsum("pij,pj->pi", A[(c_idx + 1) % sp.nC], Xh)
+    assert (np.abs(wrong - want)[inside].max(axis=1) > 1e-6).mean() > 0.9

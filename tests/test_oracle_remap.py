@@ -95,3 +95,22 @@ def test_bilinear64_exact_on_affine_images_and_matches_scipy(orc):
         d = np.zeros(2); d[k] = e
         num = (orc.bilinear64(img2, q + d)[0] - orc.bilinear64(img2, q - d)[0]) / (2 * e)
         np.testing.assert_allclose(orc.bilinear64(img2, q)[1 + k], num, rtol=1e-7)
+
+
+def test_trilinear_exact_on_affine_volumes_at_fractional_coordinates(orc):
+    """Trilinear remap at fractional coordinates (not only the identity map):
+    exact on an affine volume with a different slope per axis (a swapped or
+    transposed axis fails), and equal to scipy's order-1 map_coordinates on a
+    random volume inside."""
+    D, H, W = 12, 17, 23
+    zz, yy, xx = np.mgrid[0:D, 0:H, 0:W]
+    vol = (0.25 + 0.015625 * xx + 0.03125 * yy - 0.0625 * zz).astype(np.float32)  # exact in float32
+    rng = np.random.default_rng(5)
+    n = D * H * W
+    pts = np.stack([rng.uniform(0, W - 1, n), rng.uniform(0, H - 1, n), rng.uniform(0, D - 1, n)], 1)
+    out = orc.remap3d(pts, vol).ravel()
+    want = 0.25 + 0.015625 * pts[:, 0] + 0.03125 * pts[:, 1] - 0.0625 * pts[:, 2]
+    np.testing.assert_allclose(out, want, rtol=0, atol=4e-6)
+    v2 = rng.uniform(size=(D, H, W)).astype(np.float32)
+    ref = map_coordinates(v2.astype(np.float64), [pts[:, 2], pts[:, 1], pts[:, 0]], order=1)
+    np.testing.assert_allclose(orc.remap3d(pts, v2).ravel(), ref, rtol=1e-5, atol=2e-6)
