@@ -33,6 +33,9 @@ SIGNATURES = {
     "cpab_calc_grad_theta": [_p, _i, _i, _i, _p, _p, _p, _p, _p, _i, _d, _i, _i, _i64] + _GEO,
     "cpab_calc_T_simple_vjp": [_p, _i, _i, _i, _p, _p, _p, _d, _i, _i, _i64, _i, _i, _d, _d, _p, _p],
     "cpab_ssd_grad2d": [_p, _i, _i, _i, _p, _p, _p, _i, _i, _d, _i, _i, _i, _i, _d, _d, _p, _p, _p],
+    "cpab_calc_T_vjp": [_p, _i, _i, _i, _p, _p, _p, _p, _d, _i, _i, _i64, _i, _i, _d, _d, _p, _p],
+    "cpab_ssd_grad2d_calc_T": [_p, _i, _i, _i, _p, _p, _p, _p, _i, _i, _d, _i, _i, _i, _i, _d, _d, _p, _p, _p, _p,
+                               _p],
     "cpab_batch_create": [_p, _i, _i, _i, _p, ctypes.POINTER(_p)],
     "cpab_batch_sizes": [_p, _p, _p, _p, _p],
     "cpab_batch_prep": [_p, _p, _i, _d, _p],

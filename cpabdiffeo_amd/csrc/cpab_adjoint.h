@@ -1,31 +1,47 @@
-// Reverse-mode derivative (adjoint) of the 2D calc_T_simple map and of the
-// SSD loss of an image warped by it, with respect to the per-cell affine
-// tables.
+// Reverse-mode derivative (adjoint) of the 2D CPA maps, and of the SSD loss
+// of an image warped by them, with respect to the per-cell affine tables.
 //
-// The map is calc_T_simple (cpab/gpu/calc_2Dto2D.cu:412-445): nT*nS RK2
-// midpoint substeps, each with the cell of its start point
-// (solveODE, calc_2Dto2D.cu:167-197):
-//     v1 = A_c [p;1],  m = p + h v1 / 2,  v2 = A_c [m;1],  p' = p + v2 h.
-// It is the map the reference differentiates (its forward sensitivity
-// solveODE2, calc_2Dto2D.cu:202-319), so the adjoint here and that
-// sensitivity are two evaluations of the same derivative.  For a cotangent
-// lambda' of p' (A_c = [L | b]):
-//     g2 = h lambda',  gm = L^T g2,  g1 = (h/2) gm,
-//     lambda = lambda' + gm + L^T g1,
-//     dJ/dA_c += g2 [m;1]^T + g1 [p;1]^T.
-// (Cell changes are measure-zero events and contribute nothing, as in the
-// sensitivity.)
+// Two maps, both evaluated as the reference evaluates them:
+//  * calc_T_simple (cpab/gpu/calc_2Dto2D.cu:412-445): nT*nS RK2 midpoint
+//    substeps, each with the cell of its start point (solveODE,
+//    calc_2Dto2D.cu:167-197):
+//        v1 = A_c [p;1],  m = p + h v1 / 2,  v2 = A_c [m;1],  p' = p + v2 h.
+//    For a cotangent lambda' of p' (A_c = [L | b]):
+//        g2 = h lambda',  gm = L^T g2,  g1 = (h/2) gm,
+//        lambda = lambda' + gm + L^T g1,
+//        dJ/dA_c += g2 [m;1]^T + g1 [p;1]^T.
+//  * calc_T (calc_2Dto2D.cu:348-408), the map of calc_T_fwd / calc_T_inv:
+//    per time step the proposal pNew = Trel_c [p;1] with Trel_c =
+//    expm(dt A~_c) (the top two rows), kept when cell(pNew) == c, else the
+//    step is redone by nS RK2 substeps (as above).  An accepted step gives
+//        dJ/dTrel_c += lambda' [p;1]^T,  lambda = E_c^T lambda'
+//    (E_c the linear part of Trel_c); the gradient in the tables then
+//    follows through the matrix exponential: dJ/dA~_c += dt L(dt A~_c^T, G~)
+//    with G~ the 3x3 whose top rows are dJ/dTrel_c and L(X, E) the Frechet
+//    derivative of expm at X in the direction E (the adjoint of L(X, .) is
+//    L(X^T, .)), evaluated by k_adj_expm.
+// The accept/redo decisions and the cells of the substeps are replayed
+// from the forward sweep; they change only on a set of measure zero and
+// contribute nothing to the derivative (as in the reference's forward
+// sensitivity solveODE2, calc_2Dto2D.cu:202-319).
 //
 // Schedule, one lane per point: the forward sweep writes the point at the
-// start of every time step to a checkpoint buffer (nT points per point, not
-// the nT*nS substeps); the backward sweep walks the time steps in reverse,
-// recomputes the step's nS substep start points from its checkpoint into
-// registers (NSMAX of them, unrolled so they stay in registers), and runs
-// the substep adjoints in reverse.  dJ/dA accumulates per lane in registers
-// for the lane's current cell and is added to the block's LDS copy (one
-// nC x 6 table) when the cell changes, so LDS atomics are rare.  Per-block
-// tables and loss sums go to a partial buffer; k_adj_reduce sums them in a
-// fixed order.
+// start of every time step to a checkpoint buffer (nT points per point); the
+// backward sweep walks the time steps in reverse, recomputes the step's
+// decision from its checkpoint and, for a redone step, its nS substep start
+// points into registers (ADJ_NSMAX of them, unrolled so they stay in
+// registers), then runs the step's adjoint units in reverse: a redone step's
+// substeps j = nS-1 .. 0, an accepted step's Trel step at j = 0 (a wave
+// whose lanes all accepted runs only that unit).
+//
+// Accumulation, bit-reproducible: each lane keeps two register
+// accumulators, dJ/dA of its current RK2 cell and dJ/dTrel of its current
+// Trel cell, and flushes one when that cell changes.  Flushes go into
+// tables private to the wave (LDS), one flushing lane at a time in lane
+// order; at the end a block sums its
+// waves' tables in wave order and k_adj_reduce sums the blocks in block
+// order.  The order of every addition is fixed by the data, so the gradient
+// is the same bit for bit on every run.
 //
 // MODE 0 (dT/dA^T lambda): points and cotangents given.
 // MODE 1 (SSD): points are the H x W pixel grid (x = column, y = row); the
@@ -45,14 +61,16 @@ struct AdjDesc {
   int nInst, nC;
   int64_t nPts;          // points per instance (H*W in MODE 1)
   int H, W;
-  const double* As;      // (nInst, nC, 6): the tables the map integrates (signed As)
+  const double* As;      // (nInst, nC, 6): the signed tables the RK2 integrates
+  const double* Trels;   // (nInst, nC, 6): expm(dt As) for the calc_T map, else null
   const double* pos0;    // MODE 0: (nInst, nPts, 2)
   const double* lam;     // MODE 0: (nInst, nPts, 2) cotangents of the end points
   double* pos_out;       // optional (nInst, nPts, 2): end points
   const float* src;      // MODE 1: (nInst, H, W)
   const float* dst;      // MODE 1: (nInst, H, W)
+  uint32_t* sig;         // optional (nInst, nPts): hash of every cell the forward sweep looked up
   double* ckpt;          // (nInst, nT, nPts, 2) workspace
-  double* partial;       // (nInst, blocks per instance, nC*6 + 1)
+  double* partial;       // (nInst, blocks per instance, NG*nC*6 + 2): tables, loss, redone steps
   double h;
   int nT, nS;
   Geom g;
@@ -81,8 +99,12 @@ __device__ __forceinline__ double sample2_grad(const float* __restrict__ img, in
   return gy * a + fy * b;
 }
 
+__device__ __forceinline__ uint32_t sig_mix(uint32_t s, int c) { return s * 0x9E3779B1u + (uint32_t)(c + 1); }
+
+// One RK2 substep from p, its cell looked up at p (the same IEEE operations
+// as solveODE); returns the cell.
 template <int TESS, bool POW2>
-__device__ __forceinline__ void rk2_substep(double* p, const double* At, double h, const Geom& g) {
+__device__ __forceinline__ int rk2_substep(double* p, const double* At, double h, const Geom& g) {
   const int c = cell2<TESS, POW2>(p, g);
   const double* A = At + c * 6;
   double v[2], m[2];
@@ -92,21 +114,54 @@ __device__ __forceinline__ void rk2_substep(double* p, const double* At, double 
   affine<2>(v, A, m);
   p[0] = p[0] + v[0] * h;
   p[1] = p[1] + v[1] * h;
+  return c;
 }
 
-template <int TESS, bool POW2, int MODE>
+// Active lanes with `need` add their accumulator G into row `cell` of the
+// wave's table `tab`, one lane at a time in lane order.  Which lanes are
+// active where the call sits is fixed by the data and the code, so the order
+// of the additions is too.
+__device__ __forceinline__ void adj_flush(bool need, int cell, const double* G, double* tab) {
+  uint64_t m = __ballot(need);
+  const int lane = threadIdx.x & 63;
+  while (m) {
+    const int l = __builtin_ctzll(m);
+    if (lane == l) {
+      double* t = tab + cell * 6;
+#pragma unroll
+      for (int e = 0; e < 6; e++) t[e] = t[e] + G[e];
+    }
+    __builtin_amdgcn_wave_barrier();
+    m &= m - 1;
+  }
+}
+
+// LDS of k_adjoint2d (doubles): the tables (As, and Trels for calc_T), then
+// per wave NG gradient tables.
+__host__ __device__ inline size_t adj_lds_doubles(int nC, bool fwd, int waves) {
+  const size_t NT = (size_t)nC * 6, NG = fwd ? 2 : 1;
+  return NG * NT + (size_t)waves * NG * NT;
+}
+
+template <int TESS, bool POW2, int MODE, bool FWD>
 __global__ void __launch_bounds__(256) k_adjoint2d(AdjDesc D) {
   extern __shared__ double lds[];
   __shared__ double red[4];
+  constexpr int NG = FWD ? 2 : 1;
   const int inst = blockIdx.y;
   const int nC = D.nC, NT = nC * 6;
+  const int nW = blockDim.x >> 6, wave = threadIdx.x >> 6;
   double* At = lds;
-  double* Gs = lds + NT;
+  double* Tr = lds + NT;                        // FWD only
+  double* Gall = lds + NG * NT;                 // nW * NG tables
+  double* GA = Gall + (size_t)wave * NG * NT;   // this wave's dJ/dA
+  double* GT = GA + NT;                         // this wave's dJ/dTrel (FWD)
   const double* Ain = D.As + (int64_t)inst * NT;
   for (int i = threadIdx.x; i < NT; i += blockDim.x) {
     At[i] = Ain[i];
-    Gs[i] = 0.0;
+    if (FWD) Tr[i] = D.Trels[(int64_t)inst * NT + i];
   }
+  for (int i = threadIdx.x; i < nW * NG * NT; i += blockDim.x) Gall[i] = 0.0;
   __syncthreads();
   const Geom& g = D.g;
   const double h = D.h;
@@ -114,6 +169,7 @@ __global__ void __launch_bounds__(256) k_adjoint2d(AdjDesc D) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nPts = D.nPts;
   double loss = 0.0;
+  int nredo = 0;  // time steps redone by RK2 (every step of calc_T_simple)
   if (idx < nPts) {
     const int64_t ib = (int64_t)inst * nPts;
     double p[2];
@@ -126,11 +182,26 @@ __global__ void __launch_bounds__(256) k_adjoint2d(AdjDesc D) {
     }
     // forward sweep, one checkpoint per time step
     double* ck = D.ckpt + (int64_t)inst * nT * nPts * 2;
+    uint32_t sg = 0;
     for (int k = 0; k < nT; k++) {
       ck[((int64_t)k * nPts + idx) * 2] = p[0];
       ck[((int64_t)k * nPts + idx) * 2 + 1] = p[1];
-      for (int j = 0; j < nS; j++) rk2_substep<TESS, POW2>(p, At, h, g);
+      if (FWD) {
+        const int c = cell2<TESS, POW2>(p, g);
+        double pn[2];
+        affine<2>(pn, Tr + c * 6, p);
+        const int cn = cell2<TESS, POW2>(pn, g);
+        sg = sig_mix(sig_mix(sg, c), cn);
+        if (cn == c) {
+          p[0] = pn[0];
+          p[1] = pn[1];
+          continue;
+        }
+      }
+      nredo++;
+      for (int j = 0; j < nS; j++) sg = sig_mix(sg, rk2_substep<TESS, POW2>(p, At, h, g));
     }
+    if (D.sig) D.sig[ib + idx] = sg;
     if (D.pos_out) {
       D.pos_out[(ib + idx) * 2] = p[0];
       D.pos_out[(ib + idx) * 2 + 1] = p[1];
@@ -149,77 +220,199 @@ __global__ void __launch_bounds__(256) k_adjoint2d(AdjDesc D) {
       lam[1] = r * gy;
     }
     // backward sweep
-    int cl = -1;
-    double G[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int ca = -1, ct = -1;  // cells of the RK2 and Trel accumulators (-1: empty)
+    double Ga[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double Gt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int k = nT - 1; k >= 0; k--) {
+      const double q[2] = {ck[((int64_t)k * nPts + idx) * 2], ck[((int64_t)k * nPts + idx) * 2 + 1]};
+      bool redo = true;
+      int c0 = 0;
+      if (FWD) {
+        c0 = cell2<TESS, POW2>(q, g);
+        double pn[2];
+        affine<2>(pn, Tr + c0 * 6, q);
+        redo = cell2<TESS, POW2>(pn, g) != c0;
+      }
+      const bool any_redo = FWD ? (__ballot(redo) != 0) : true;  // wave-uniform
       double ps[ADJ_NSMAX][2];
-      double q[2] = {ck[((int64_t)k * nPts + idx) * 2], ck[((int64_t)k * nPts + idx) * 2 + 1]};
+      if (any_redo) {
+        double r[2] = {q[0], q[1]};
 #pragma unroll
-      for (int j = 0; j < ADJ_NSMAX; j++) {
-        if (j < nS) {
-          ps[j][0] = q[0];
-          ps[j][1] = q[1];
-          rk2_substep<TESS, POW2>(q, At, h, g);
+        for (int j = 0; j < ADJ_NSMAX; j++) {
+          if (j < nS) {
+            ps[j][0] = r[0];
+            ps[j][1] = r[1];
+            if (redo) rk2_substep<TESS, POW2>(r, At, h, g);
+          }
         }
       }
 #pragma unroll
       for (int j = ADJ_NSMAX - 1; j >= 0; j--) {
-        if (j < nS) {
-          const double* pj = ps[j];
-          const int c = cell2<TESS, POW2>(pj, g);
-          const double* A = At + c * 6;
-          double v1[2], m[2];
-          affine<2>(v1, A, pj);
-          m[0] = pj[0] + h * v1[0] / 2.;
-          m[1] = pj[1] + h * v1[1] / 2.;
-          const double g2x = h * lam[0], g2y = h * lam[1];
-          const double gmx = A[0] * g2x + A[3] * g2y, gmy = A[1] * g2x + A[4] * g2y;
-          const double g1x = 0.5 * h * gmx, g1y = 0.5 * h * gmy;
-          lam[0] = lam[0] + gmx + (A[0] * g1x + A[3] * g1y);
-          lam[1] = lam[1] + gmy + (A[1] * g1x + A[4] * g1y);
-          if (c != cl) {
-            if (cl >= 0)
+        if (j < nS && (any_redo || j == 0)) {  // wave-uniform
+          if (redo) {  // substep j of a redone step
+            const double* pj = ps[j];
+            const int c = cell2<TESS, POW2>(pj, g);
+            adj_flush(ca >= 0 && c != ca, ca, Ga, GA);
+            if (c != ca) {
 #pragma unroll
-              for (int e = 0; e < 6; e++) atomicAdd(Gs + cl * 6 + e, G[e]);
+              for (int e = 0; e < 6; e++) Ga[e] = 0.0;
+              ca = c;
+            }
+            const double* A = At + c * 6;
+            double v1[2], m[2];
+            affine<2>(v1, A, pj);
+            m[0] = pj[0] + h * v1[0] / 2.;
+            m[1] = pj[1] + h * v1[1] / 2.;
+            const double g2x = h * lam[0], g2y = h * lam[1];
+            const double gmx = A[0] * g2x + A[3] * g2y, gmy = A[1] * g2x + A[4] * g2y;
+            const double g1x = 0.5 * h * gmx, g1y = 0.5 * h * gmy;
+            lam[0] = lam[0] + gmx + (A[0] * g1x + A[3] * g1y);
+            lam[1] = lam[1] + gmy + (A[1] * g1x + A[4] * g1y);
+            Ga[0] += g2x * m[0] + g1x * pj[0];
+            Ga[1] += g2x * m[1] + g1x * pj[1];
+            Ga[2] += g2x + g1x;
+            Ga[3] += g2y * m[0] + g1y * pj[0];
+            Ga[4] += g2y * m[1] + g1y * pj[1];
+            Ga[5] += g2y + g1y;
+          } else if (FWD && j == 0) {  // the accepted Trel step
+            adj_flush(ct >= 0 && c0 != ct, ct, Gt, GT);
+            if (c0 != ct) {
 #pragma unroll
-            for (int e = 0; e < 6; e++) G[e] = 0.0;
-            cl = c;
+              for (int e = 0; e < 6; e++) Gt[e] = 0.0;
+              ct = c0;
+            }
+            const double* T = Tr + c0 * 6;
+            Gt[0] += lam[0] * q[0];
+            Gt[1] += lam[0] * q[1];
+            Gt[2] += lam[0];
+            Gt[3] += lam[1] * q[0];
+            Gt[4] += lam[1] * q[1];
+            Gt[5] += lam[1];
+            const double l0 = T[0] * lam[0] + T[3] * lam[1];
+            const double l1 = T[1] * lam[0] + T[4] * lam[1];
+            lam[0] = l0;
+            lam[1] = l1;
           }
-          G[0] += g2x * m[0] + g1x * pj[0];
-          G[1] += g2x * m[1] + g1x * pj[1];
-          G[2] += g2x + g1x;
-          G[3] += g2y * m[0] + g1y * pj[0];
-          G[4] += g2y * m[1] + g1y * pj[1];
-          G[5] += g2y + g1y;
         }
       }
     }
-    if (cl >= 0)
-#pragma unroll
-      for (int e = 0; e < 6; e++) atomicAdd(Gs + cl * 6 + e, G[e]);
+    adj_flush(ca >= 0, ca, Ga, GA);
+    if (FWD) adj_flush(ct >= 0, ct, Gt, GT);
   }
   const double ls = block_sum(loss, red);
   __syncthreads();
-  double* out = D.partial + ((int64_t)inst * gridDim.x + blockIdx.x) * (NT + 1);
-  for (int i = threadIdx.x; i < NT; i += blockDim.x) out[i] = Gs[i];
-  if (threadIdx.x == 0) out[NT] = ls;
+  const double nr = block_sum((double)nredo, red);
+  __syncthreads();
+  // the block's tables: its waves' tables summed in wave order
+  double* out = D.partial + ((int64_t)inst * gridDim.x + blockIdx.x) * (NG * NT + 2);
+  for (int i = threadIdx.x; i < NG * NT; i += blockDim.x) {
+    double s = 0.0;
+    for (int w = 0; w < nW; w++) s += Gall[(size_t)w * NG * NT + i];
+    out[i] = s;
+  }
+  if (threadIdx.x == 0) {
+    out[NG * NT] = ls;
+    out[NG * NT + 1] = nr;
+  }
 }
 
 // Sum the per-block partials of every instance in a fixed order:
-// gradA (nInst, nC*6) and loss (nInst,).
-__global__ void k_adj_reduce(const double* __restrict__ partial, int nBlocks, int NT, double* __restrict__ gradA,
-                             double* __restrict__ loss) {
+// gradA (nInst, nC*6), gradT (nInst, nC*6; calc_T map only), loss (nInst,)
+// and redone steps (nInst,), the last two optional.
+__global__ void k_adj_reduce(const double* __restrict__ partial, int nBlocks, int NT, int NG,
+                             double* __restrict__ gradA, double* __restrict__ gradT, double* __restrict__ loss,
+                             double* __restrict__ nredo) {
   const int inst = blockIdx.x;
-  const double* pi = partial + (int64_t)inst * nBlocks * (NT + 1);
-  for (int e = threadIdx.x; e <= NT; e += blockDim.x) {
+  const int stride = NG * NT + 2;
+  const double* pi = partial + (int64_t)inst * nBlocks * stride;
+  for (int e = threadIdx.x; e < stride; e += blockDim.x) {
     double s = 0.0;
-    for (int b = 0; b < nBlocks; b++) s += pi[(int64_t)b * (NT + 1) + e];
+    for (int b = 0; b < nBlocks; b++) s += pi[(int64_t)b * stride + e];
     if (e < NT) {
       gradA[(int64_t)inst * NT + e] = s;
-    } else if (loss) {
-      loss[inst] = s;
+    } else if (e < NG * NT) {
+      gradT[(int64_t)inst * NT + e - NT] = s;
+    } else if (e == NG * NT) {
+      if (loss) loss[inst] = s;
+    } else if (nredo) {
+      nredo[inst] = s;
     }
   }
+}
+
+// 3x3 row-major products.
+__device__ inline void mat3_mul(const double* a, const double* b, double* c) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) c[i * 3 + j] = a[i * 3] * b[j] + a[i * 3 + 1] * b[3 + j] + a[i * 3 + 2] * b[6 + j];
+}
+
+// The Frechet derivative L(X, E) of the matrix exponential (3x3): the upper
+// right block of exp([[X, E], [0, X]]) (Van Loan's block form), by scaling
+// and squaring with a degree-20 Taylor polynomial in Horner form on the
+// block pairs (D, U) ~ [[D, U], [0, D]].  E is normalised first (L is
+// linear in E), so the scaling follows the size of X alone.
+__device__ inline void expm_frechet3(const double* X, const double* E, double* L) {
+  double en = 0.0, xn = 0.0;
+  for (int j = 0; j < 3; j++) {
+    double se = 0.0, sx = 0.0;
+    for (int i = 0; i < 3; i++) {
+      se += fabs(E[i * 3 + j]);
+      sx += fabs(X[i * 3 + j]);
+    }
+    en = se > en ? se : en;
+    xn = sx > xn ? sx : xn;
+  }
+  for (int i = 0; i < 9; i++) L[i] = 0.0;
+  if (en == 0.0) return;
+  int s = 0;
+  while (xn > 0.5 && s < 64) {
+    xn *= 0.5;
+    s++;
+  }
+  const double sc = ldexp(1.0, -s);
+  double x[9], e[9], Dm[9], U[9], t1[9], t2[9];
+  for (int i = 0; i < 9; i++) {
+    x[i] = X[i] * sc;
+    e[i] = E[i] / en * sc;
+    Dm[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    U[i] = 0.0;
+  }
+  for (int k = 20; k >= 1; k--) {
+    // (D, U) <- (I + x D / k, (x U + e D) / k)
+    mat3_mul(x, U, t1);
+    mat3_mul(e, Dm, t2);
+    for (int i = 0; i < 9; i++) U[i] = (t1[i] + t2[i]) / (double)k;
+    mat3_mul(x, Dm, t1);
+    for (int i = 0; i < 9; i++) Dm[i] = ((i % 4 == 0) ? 1.0 : 0.0) + t1[i] / (double)k;
+  }
+  for (int q = 0; q < s; q++) {
+    // (D, U) <- (D D, D U + U D)
+    mat3_mul(Dm, U, t1);
+    mat3_mul(U, Dm, t2);
+    for (int i = 0; i < 9; i++) U[i] = t1[i] + t2[i];
+    mat3_mul(Dm, Dm, t1);
+    for (int i = 0; i < 9; i++) Dm[i] = t1[i];
+  }
+  for (int i = 0; i < 9; i++) L[i] = U[i] * en;
+}
+
+// gradA_c += dt * (top rows of L(dt A~_c^T, G~_c)), G~_c the 3x3 with the
+// rows dJ/dTrel_c: the chain rule through Trel_c = expm(dt A~_c).  One lane
+// per (instance, cell).
+__global__ void k_adj_expm(const double* __restrict__ As, const double* __restrict__ gradT, double* __restrict__ gradA,
+                           int nC, int nInst, double dt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nC * nInst) return;
+  const double* a = As + i * 6;
+  const double* gt = gradT + i * 6;
+  const double X[9] = {dt * a[0], dt * a[3], 0.0, dt * a[1], dt * a[4], 0.0, dt * a[2], dt * a[5], 0.0};
+  const double E[9] = {gt[0], gt[1], gt[2], gt[3], gt[4], gt[5], 0.0, 0.0, 0.0};
+  double L[9];
+  expm_frechet3(X, E, L);
+  double* ga = gradA + i * 6;
+  for (int e = 0; e < 6; e++) ga[e] = ga[e] + dt * L[e];
 }
 
 }  // namespace cpab

@@ -594,10 +594,13 @@ int ensure_ws(cpab_ctx* ctx, size_t bytes) {
   return CPAB_OK;
 }
 
-// Shared by the two adjoint entries: checks, workspace, launch, reduction.
+// Shared by the adjoint entries: checks, workspace, launch, reductions.
+// Trels non-null: the calc_T map (Trel proposals + RK2 redos), else
+// calc_T_simple's RK2.
 int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts, int H, int W, const double* pos0,
-              const double* lam, const float* src, const float* dst, const double* As, double dt, int nT, int nS,
-              int nC0, int nC1, double ix, double iy, double* pos_out, double* loss, double* gradA) {
+              const double* lam, const float* src, const float* dst, const double* Trels, const double* As, double dt,
+              int nT, int nS, int nC0, int nC1, double ix, double iy, double* pos_out, double* loss, double* gradA,
+              uint32_t* sig, double* nredo) {
   Geom g;
   bool pow2;
   int r = make_geom(ctx, 2, tess, nC0, nC1, 0, ix, iy, 1.0, &g, &pow2);
@@ -607,8 +610,12 @@ int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts
   if (nPts < 0 || nT < 0 || nS < 0) return fail(ctx, CPAB_ERR_ARG, "bad sizes");
   if (nS > ADJ_NSMAX) return fail(ctx, CPAB_ERR_UNSUPPORTED, "nStepsODE > %d is not supported by the adjoint", ADJ_NSMAX);
   if (!As || !gradA) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
-  const int NT = nC * 6;
-  const size_t lds = (size_t)2 * NT * sizeof(double);
+  const bool fwd = Trels != nullptr;
+  const int NT = nC * 6, NG = fwd ? 2 : 1;
+  // block of 4, 2 or 1 waves: the per-wave gradient tables must fit in LDS
+  int waves = 4;
+  while (waves > 1 && adj_lds_doubles(nC, fwd, waves) * sizeof(double) > (size_t)ctx->max_lds) waves >>= 1;
+  const size_t lds = adj_lds_doubles(nC, fwd, waves) * sizeof(double);
   if (lds > (size_t)ctx->max_lds) return fail(ctx, CPAB_ERR_UNSUPPORTED, "tables exceed LDS");
   if (nPts == 0 || nT == 0) {  // the identity map: zero gradient (and loss of the unwarped images in MODE 1)
     if (mode == 1) return fail(ctx, CPAB_ERR_UNSUPPORTED, "empty integration in the SSD adjoint");
@@ -618,10 +625,12 @@ int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts
                                  ctx->stream));
     return CPAB_OK;
   }
-  const int64_t nbx = (nPts + 255) / 256;
+  const int block = 64 * waves;
+  const int64_t nbx = (nPts + block - 1) / block;
   if (nbx > 0x7fffffffLL) return fail(ctx, CPAB_ERR_ARG, "too many points");
-  const size_t ck = (size_t)nInst * nT * nPts * 2, pa = (size_t)nInst * nbx * (NT + 1);
-  if ((r = ensure_ws(ctx, (ck + pa) * sizeof(double)))) return r;
+  const size_t ck = (size_t)nInst * nT * nPts * 2, pa = (size_t)nInst * nbx * (NG * NT + 2);
+  const size_t gt = fwd ? (size_t)nInst * NT : 0;
+  if ((r = ensure_ws(ctx, (ck + pa + gt) * sizeof(double)))) return r;
   AdjDesc D{};
   D.nInst = nInst;
   D.nC = nC;
@@ -629,11 +638,13 @@ int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts
   D.H = H;
   D.W = W;
   D.As = As;
+  D.Trels = Trels;
   D.pos0 = pos0;
   D.lam = lam;
   D.pos_out = pos_out;
   D.src = src;
   D.dst = dst;
+  D.sig = sig;
   D.ckpt = ctx->ws;
   D.partial = ctx->ws + ck;
   D.h = dt / (double)nS;
@@ -643,17 +654,22 @@ int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts
   const dim3 grid((unsigned)nbx, (unsigned)nInst);
   hipEvent_t a = nullptr, b = nullptr;
   if ((r = prof_begin(ctx, &a, &b))) return r;
-#define ADJ_LAUNCH(T_, P_)                                                                          \
-  do {                                                                                              \
-    if (mode == 0) {                                                                                \
-      auto k = k_adjoint2d<T_, P_, 0>;                                                              \
-      allow_lds(k, lds);                                                                            \
-      hipLaunchKernelGGL(k, grid, dim3(256), lds, ctx->stream, D);                                  \
-    } else {                                                                                        \
-      auto k = k_adjoint2d<T_, P_, 1>;                                                              \
-      allow_lds(k, lds);                                                                            \
-      hipLaunchKernelGGL(k, grid, dim3(256), lds, ctx->stream, D);                                  \
-    }                                                                                               \
+#define ADJ_LAUNCH3(T_, P_, M_)                                             \
+  do {                                                                     \
+    if (fwd) {                                                             \
+      auto k = k_adjoint2d<T_, P_, M_, true>;                              \
+      allow_lds(k, lds);                                                   \
+      hipLaunchKernelGGL(k, grid, dim3(block), lds, ctx->stream, D);       \
+    } else {                                                               \
+      auto k = k_adjoint2d<T_, P_, M_, false>;                             \
+      allow_lds(k, lds);                                                   \
+      hipLaunchKernelGGL(k, grid, dim3(block), lds, ctx->stream, D);       \
+    }                                                                      \
+  } while (0)
+#define ADJ_LAUNCH(T_, P_)                   \
+  do {                                       \
+    if (mode == 0) ADJ_LAUNCH3(T_, P_, 0);   \
+    else ADJ_LAUNCH3(T_, P_, 1);             \
   } while (0)
   if (tess == TESS_I) {
     if (pow2) ADJ_LAUNCH(TESS_I, true);
@@ -663,9 +679,17 @@ int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts
     else ADJ_LAUNCH(TESS_II, false);
   }
 #undef ADJ_LAUNCH
+#undef ADJ_LAUNCH3
   HIP_OK(ctx, hipGetLastError());
   if ((r = prof_end(ctx, a, b))) return r;
-  hipLaunchKernelGGL(k_adj_reduce, dim3(nInst), dim3(256), 0, ctx->stream, D.partial, (int)nbx, NT, gradA, loss);
+  double* gradT = fwd ? ctx->ws + ck + pa : nullptr;
+  hipLaunchKernelGGL(k_adj_reduce, dim3(nInst), dim3(256), 0, ctx->stream, D.partial, (int)nbx, NT, NG, gradA, gradT,
+                     loss, nredo);
+  if (fwd) {
+    const int64_t n = (int64_t)nInst * nC;
+    hipLaunchKernelGGL(k_adj_expm, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, ctx->stream, As, gradT, gradA, nC,
+                       nInst, dt);
+  }
   HIP_OK(ctx, hipGetLastError());
   return CPAB_OK;
 }
@@ -679,8 +703,8 @@ int cpab_calc_T_simple_vjp(cpab_ctx* ctx, int tess, int nC, int nInst, const dou
                            double inc_x, double inc_y, double* pos_out, double* gradA) {
   if (!ctx) return CPAB_ERR_ARG;
   if (nPts > 0 && (!pos0 || !lam)) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
-  return adjoint2d(ctx, 0, tess, nC, nInst, nPts, 0, 0, pos0, lam, nullptr, nullptr, As, dt, nTimeSteps, nStepsODE,
-                   nC0, nC1, inc_x, inc_y, pos_out, nullptr, gradA);
+  return adjoint2d(ctx, 0, tess, nC, nInst, nPts, 0, 0, pos0, lam, nullptr, nullptr, nullptr, As, dt, nTimeSteps,
+                   nStepsODE, nC0, nC1, inc_x, inc_y, pos_out, nullptr, gradA, nullptr, nullptr);
 }
 
 int cpab_ssd_grad2d(cpab_ctx* ctx, int tess, int nC, int nInst, const double* As, const float* src, const float* dst,
@@ -689,8 +713,29 @@ int cpab_ssd_grad2d(cpab_ctx* ctx, int tess, int nC, int nInst, const double* As
   if (!ctx) return CPAB_ERR_ARG;
   if (H < 1 || W < 1) return fail(ctx, CPAB_ERR_ARG, "images must be non-empty");
   if (!src || !dst || !loss) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
-  return adjoint2d(ctx, 1, tess, nC, nInst, (int64_t)H * W, H, W, nullptr, nullptr, src, dst, As, dt, nTimeSteps,
-                   nStepsODE, nC0, nC1, inc_x, inc_y, pos_out, loss, gradA);
+  return adjoint2d(ctx, 1, tess, nC, nInst, (int64_t)H * W, H, W, nullptr, nullptr, src, dst, nullptr, As, dt,
+                   nTimeSteps, nStepsODE, nC0, nC1, inc_x, inc_y, pos_out, loss, gradA, nullptr, nullptr);
+}
+
+int cpab_calc_T_vjp(cpab_ctx* ctx, int tess, int nC, int nInst, const double* pos0, const double* lam,
+                    const double* Trels, const double* As, double dt, int nTimeSteps, int nStepsODE, int64_t nPts,
+                    int nC0, int nC1, double inc_x, double inc_y, double* pos_out, double* gradA) {
+  if (!ctx) return CPAB_ERR_ARG;
+  if (!Trels) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  if (nPts > 0 && (!pos0 || !lam)) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  return adjoint2d(ctx, 0, tess, nC, nInst, nPts, 0, 0, pos0, lam, nullptr, nullptr, Trels, As, dt, nTimeSteps,
+                   nStepsODE, nC0, nC1, inc_x, inc_y, pos_out, nullptr, gradA, nullptr, nullptr);
+}
+
+int cpab_ssd_grad2d_calc_T(cpab_ctx* ctx, int tess, int nC, int nInst, const double* Trels, const double* As,
+                           const float* src, const float* dst, int H, int W, double dt, int nTimeSteps, int nStepsODE,
+                           int nC0, int nC1, double inc_x, double inc_y, double* pos_out, double* loss,
+                           double* gradA, uint32_t* sig, double* n_redo) {
+  if (!ctx) return CPAB_ERR_ARG;
+  if (H < 1 || W < 1) return fail(ctx, CPAB_ERR_ARG, "images must be non-empty");
+  if (!src || !dst || !loss || !Trels) return fail(ctx, CPAB_ERR_ARG, "null pointer argument");
+  return adjoint2d(ctx, 1, tess, nC, nInst, (int64_t)H * W, H, W, nullptr, nullptr, src, dst, Trels, As, dt,
+                   nTimeSteps, nStepsODE, nC0, nC1, inc_x, inc_y, pos_out, loss, gradA, sig, n_redo);
 }
 
 // ------------------------------------------------------------- batch

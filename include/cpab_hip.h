@@ -127,6 +127,35 @@ int cpab_ssd_grad2d(cpab_ctx *ctx, int tess, int nC, int nInst, const double *As
                     int nC0, int nC1, double inc_x, double inc_y, double *pos_out, double *loss,
                     double *gradA);
 
+/* Reverse mode of calc_T (the map of calc_T_fwd / calc_T_inv,
+ * cpab/gpu/calc_2Dto2D.cu:348-408; bound nowhere in the reference, which
+ * differentiates only calc_T_simple): per time step the Trel proposal, kept
+ * when it stays in its cell, else nStepsODE RK2 substeps, with the decisions
+ * replayed from the forward sweep.  Trels = expm(dt As) as cpab_prep_trels
+ * forms them (cpab/gpu/expm/expm_affine_by_series_2D.py:85-237), As the
+ * signed tables (nInst, nC, 6).  gradA = sum_x lam(x)^T dT(x)/dAs, through
+ * the Trel steps (the adjoint of the Frechet derivative of expm) and the RK2
+ * substeps.  The additions follow a fixed order: gradA is the same bit for bit
+ * on every run.  Same checkpoint workspace and limits as
+ * cpab_calc_T_simple_vjp. */
+int cpab_calc_T_vjp(cpab_ctx *ctx, int tess, int nC, int nInst, const double *pos0,
+                    const double *lam, const double *Trels, const double *As, double dt,
+                    int nTimeSteps, int nStepsODE, int64_t nPts, int nC0, int nC1, double inc_x,
+                    double inc_y, double *pos_out, double *gradA);
+
+/* cpab_ssd_grad2d for the calc_T map (Trels as in cpab_calc_T_vjp): the SSD
+ * loss of the images warped by calc_T and its gradient in the signed tables.
+ * sig (optional, uint32 (nInst, H*W)): a hash of every cell the forward
+ * sweep looked up per pixel (equal hashes under two parameter values: the
+ * same accept/redo decisions and substep cells, so the map is smooth
+ * between them).  n_redo (optional, (nInst,)): the time steps redone by RK2,
+ * summed over the pixels. */
+int cpab_ssd_grad2d_calc_T(cpab_ctx *ctx, int tess, int nC, int nInst, const double *Trels,
+                           const double *As, const float *src, const float *dst, int H, int W,
+                           double dt, int nTimeSteps, int nStepsODE, int nC0, int nC1,
+                           double inc_x, double inc_y, double *pos_out, double *loss,
+                           double *gradA, uint32_t *sig, double *n_redo);
+
 /* calc_v: Calcs.calc_velocities (cpab/gpu/Calcs.py:816-924), kernel
  * cpab/gpu/calc_2Dto2D.cu:561-602.  As is the unsigned table. */
 int cpab_calc_v(cpab_ctx *ctx, int dim, int tess, int nC, const double *pts, double *vel,

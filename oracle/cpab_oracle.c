@@ -436,6 +436,75 @@ void oracle_calc_grad_theta(int dim, int tess, const double *pos0, double *pos, 
     }
 }
 
+/* Forward sensitivity of calc_T (calc_2Dto2D.cu:348-408), the map the
+ * reference integrates for calc_T_fwd / calc_T_inv: per time step the Trel
+ * proposal pNew = Trel_c [p;1], kept when its cell is c, else nStepsODE RK2
+ * substeps.  With the decisions of the step held (they change only on a set
+ * of measure zero), an accepted step propagates q_j' = E_c q_j + dT_jc [p;1]
+ * (E_c the linear part of Trel_c, dT_jc = dTrel_c / dtheta_j, layout
+ * (d, nC, dim, dim+1) like Bs), a redone step the RK2 substeps of
+ * solveODE2 (calc_2Dto2D.cu:202-319) with Bs = dAs / dtheta.  grad has
+ * layout (nPts, dim, d); pos receives calc_T's end points.  No reference
+ * counterpart computes this derivative (the reference differentiates only
+ * calc_T_simple's RK2). */
+void oracle_calc_grad_theta_T(int dim, int tess, const double *pos0, double *pos,
+                              const double *Trels, const double *As, const double *Bs,
+                              const double *dTs, double *grad, int d, int nC, double dt,
+                              int nTimeSteps, int nStepsODE, int64_t nPts, int nC0, int nC1,
+                              int nC2, double inc_x, double inc_y, double inc_z, int nthreads) {
+    geom_t g;
+    fill_geom(&g, dim, tess, nC0, nC1, nC2, inc_x, inc_y, inc_z);
+    const int ne = dim * (dim + 1);
+#pragma omp parallel for schedule(static) num_threads(pick_threads(nthreads))
+    for (int64_t idx = 0; idx < nPts; idx++) {
+        double p[3], pNew[3], v[3], pMid[3], vMid[3], q[3], qn[3], qMid[3], u[3], uMid[3], BtT[3],
+            AdT[3];
+        for (int i = 0; i < dim; ++i) p[i] = pos0[idx * dim + i];
+        double h = dt / (double)nStepsODE;
+        double *gp = grad + idx * (int64_t)dim * d;
+        for (int j = 0; j < d; j++)
+            for (int i = 0; i < dim; ++i) gp[i * d + j] = 0;
+        for (int t = 0; t < nTimeSteps; ++t) {
+            int c = cell_of(p, &g);
+            const double *T = Trels + (size_t)c * ne;
+            A_times_b_affine(dim, pNew, T, p);
+            if (cell_of(pNew, &g) == c) {
+                for (int j = 0; j < d; j++) {
+                    int64_t bi = (int64_t)j * ne * nC + (int64_t)c * ne;
+                    for (int i = 0; i < dim; ++i) q[i] = gp[i * d + j];
+                    A_times_b_linear(dim, qn, T, q);
+                    A_times_b_affine(dim, BtT, dTs + bi, p);
+                    for (int i = 0; i < dim; ++i) gp[i * d + j] = qn[i] + BtT[i];
+                }
+                for (int i = 0; i < dim; ++i) p[i] = pNew[i];
+                continue;
+            }
+            for (int s = 0; s < nStepsODE; ++s) {
+                int cs = cell_of(p, &g);
+                int mi = cs * ne;
+                A_times_b_affine(dim, v, As + mi, p);
+                for (int i = 0; i < dim; ++i) pMid[i] = p[i] + h * v[i] / 2.;
+                A_times_b_affine(dim, vMid, As + mi, pMid);
+                for (int j = 0; j < d; j++) {
+                    int64_t bi = (int64_t)j * ne * nC + mi;
+                    for (int i = 0; i < dim; ++i) q[i] = gp[i * d + j];
+                    A_times_b_affine(dim, BtT, Bs + bi, p);
+                    A_times_b_linear(dim, AdT, As + mi, q);
+                    for (int i = 0; i < dim; ++i) u[i] = BtT[i] + AdT[i];
+                    for (int i = 0; i < dim; ++i) qMid[i] = q[i] + h * u[i] / 2.;
+                    A_times_b_affine(dim, BtT, Bs + bi, pMid);
+                    A_times_b_linear(dim, AdT, As + mi, qMid);
+                    for (int i = 0; i < dim; ++i) uMid[i] = BtT[i] + AdT[i];
+                    for (int i = 0; i < dim; ++i) q[i] += uMid[i] * h;
+                    for (int i = 0; i < dim; ++i) gp[i * d + j] = q[i];
+                }
+                for (int i = 0; i < dim; ++i) p[i] += vMid[i] * h;
+            }
+        }
+        for (int i = 0; i < dim; ++i) pos[idx * dim + i] = p[i];
+    }
+}
+
 /* --------------------------------------------------------------- expm */
 
 /* expm_affine_by_series_2D.py:28-74 */

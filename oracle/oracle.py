@@ -47,6 +47,8 @@ def lib():
         L.oracle_facet_dist.argtypes = [_i, _i, _p, _p, _i64] + geo
         L.oracle_calc_grad_theta.argtypes = ([_i, _i, _p, _p, _p, _p, _p, _i, _i, _d, _i, _i, _i64]
                                              + geo + [_i])
+        L.oracle_calc_grad_theta_T.argtypes = ([_i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i, _d, _i, _i, _i64]
+                                               + geo + [_i])
         L.oracle_prep.argtypes = [_i, _i, _p, _i, _d, _p, _p]
         L.oracle_expm.argtypes = [_i, _i, _p, _p, _i]
         L.oracle_remap2d.argtypes = [_p, _p, _p, _i, _i, _i]
@@ -171,6 +173,49 @@ def calc_grad_theta(geom, pos0, As, Bs, dt, nTimeSteps, nStepsODE, nthreads=0):
     lib().oracle_calc_grad_theta(geom["dim"], geom["tess"], p0, pos.ctypes.data_as(_p), pA, pB,
                                  grad.ctypes.data_as(_p), d, nC, float(dt), int(nTimeSteps),
                                  int(nStepsODE), nPts, *_geo(geom), int(nthreads))
+    return pos, grad
+
+
+def trel_sensitivity(dim, sA, Bs, dt):
+    """dTrel_c / dtheta_j = the top dim rows of the Frechet derivative of the
+    matrix exponential at dt * A~_c in the direction dt * B~_jc (A~, B~ the
+    (dim+1) x (dim+1) affine matrices with a zero last row), computed by
+    scipy.linalg.expm_frechet (Al-Mohy and Higham, SIAM J. Matrix Anal.
+    Appl. 30(4), 2009), independent of any code here.  sA (nC, dim(dim+1)):
+    the signed tables Trel = expm(dt sA); Bs (d, nC, dim, dim+1) = dsA /
+    dtheta.  Returns dTs with Bs's layout."""
+    from scipy.linalg import expm_frechet
+    sA = np.asarray(sA, dtype=np.float64).reshape(-1, dim, dim + 1)
+    Bs = np.asarray(Bs, dtype=np.float64)
+    d, nC = Bs.shape[0], Bs.shape[1]
+    out = np.zeros((d, nC, dim, dim + 1))
+    X = np.zeros((dim + 1, dim + 1))
+    E = np.zeros((dim + 1, dim + 1))
+    for c in range(nC):
+        X[:dim] = dt * sA[c]
+        for j in range(d):
+            E[:dim] = dt * Bs[j, c]
+            out[j, c] = expm_frechet(X, E, compute_expm=False)[:dim]
+    return out
+
+
+def calc_grad_theta_T(geom, pos0, Trels, As, Bs, dTs, dt, nTimeSteps, nStepsODE, nthreads=0):
+    """Forward sensitivity of calc_T (accepted Trel steps and RK2 redos, the
+    decisions of calc_2Dto2D.cu:348-408 held).  Bs, dTs: (d, nC, dim, dim+1)
+    derivatives of the signed tables and of the Trels in theta.  Returns
+    (calc_T end points, grad (nPts, dim, d))."""
+    pos0, p0 = _c(pos0, np.float64)
+    nPts, dim = pos0.shape
+    Bs, pB = _c(Bs, np.float64)
+    dTs, pD = _c(dTs, np.float64)
+    d, nC = Bs.shape[0], Bs.shape[1]
+    As, pA = _c(As, np.float64)
+    Trels, pT = _c(Trels, np.float64)
+    pos = np.empty_like(pos0)
+    grad = np.empty((nPts, dim, d), np.float64)
+    lib().oracle_calc_grad_theta_T(geom["dim"], geom["tess"], p0, pos.ctypes.data_as(_p), pT, pA, pB, pD,
+                                   grad.ctypes.data_as(_p), d, nC, float(dt), int(nTimeSteps),
+                                   int(nStepsODE), nPts, *_geo(geom), int(nthreads))
     return pos, grad
 
 

@@ -157,6 +157,44 @@ def test_sensitivity_matches_finite_differences(orc):
     assert (err < 1e-6).mean() > 0.9, np.sort(err)[-5:]
 
 
+@pytest.mark.parametrize("sign", [1, -1])
+def test_calc_T_sensitivity_matches_finite_differences(orc, sign):
+    """The oracle's forward sensitivity of calc_T (accepted Trel steps through
+    the Frechet derivative of expm, redone steps through the RK2 substeps)
+    against central differences of the oracle's own calc_T in theta, on
+    points whose every lookup stays farther than 1e-4 from a facet (their
+    accept/redo decisions and substep cells cannot change under +-h)."""
+    sp = configs.unit_space((2, 2), "I")
+    rng = np.random.default_rng(9)
+    theta = rng.normal(size=sp.d) * 2.0
+    Av = sp.B @ theta
+    g = sp.tessellation.geom()
+    pts = rng.uniform(0.02, 0.98, (300, 2))
+    dt, nT, nS = 0.05, 20, 5
+    sA, T = tables(orc, sp, Av, sign, dt)
+    Bs = sign * sp.BasMats
+    dTs = orc.trel_sensitivity(2, sA, Bs, dt)
+    pos, grad = orc.calc_grad_theta_T(g, pts, T, sA, Bs, dTs, dt, nT, nS)
+    end, nfb, md = orc.calc_T(g, pts, T, sA, dt, nT, nS, probe=True)
+    assert np.array_equal(pos, end)
+    h = 1e-6
+    fd = np.zeros_like(grad)
+    for j in range(sp.d):
+        e = np.zeros(sp.d)
+        e[j] = h
+        sp_, Tp = tables(orc, sp, sp.B @ (theta + e), sign, dt)
+        sm_, Tm = tables(orc, sp, sp.B @ (theta - e), sign, dt)
+        fd[:, :, j] = (orc.calc_T(g, pts, Tp, sp_, dt, nT, nS) - orc.calc_T(g, pts, Tm, sm_, dt, nT, nS)) / (2 * h)
+    ok = md > 1e-4
+    scale = np.abs(grad).max(axis=(1, 2))
+    err = np.abs(fd - grad).max(axis=(1, 2)) / scale
+    assert ok.sum() > 150 and (nfb[ok] > 0).sum() > 20  # both kinds of step are exercised
+    assert err[ok].max() < 1e-6, np.sort(err[ok])[-5:]
+    # the Trel step's own derivative matters: dropping it breaks the match
+    pos0, grad0 = orc.calc_grad_theta_T(g, pts, T, sA, Bs, np.zeros_like(dTs), dt, nT, nS)
+    assert (np.abs(fd - grad0).max(axis=(1, 2)) / scale)[ok].min() > 1e-3
+
+
 def _bary_velocity(sp, Avees, X):
     """v(X) = A_c [X; 1] with the simplex c found by barycentric coordinates
     (largest smallest coordinate), any dimension: independent of the
