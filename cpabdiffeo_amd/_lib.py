@@ -42,6 +42,7 @@ SIGNATURES = {
     "cpab_batch_calc_T": [_p, _p, _p, _p, _d, _i, _i, _p, _i, _p],
     "cpab_batch_destroy": [_p],
     "cpab_calc_T_items": [_p, _i, _i, _i, _p, _d, _i, _i, _p],
+    "cpab_calc_T_items_noalloc": [_p, _i, _i, _i, _p, _d, _i, _i, _p],
     "cpab_theta2avees": [_p, _i, _i, _i, _p, _p, _p],
     "cpab_calc_v": [_p, _i, _i, _i, _p, _p, _p, _i64] + _GEO,
     "cpab_calc_cell_idx": [_p, _i, _i, _p, _p, _i64] + _GEO,

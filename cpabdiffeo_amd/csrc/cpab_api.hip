@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -42,7 +43,22 @@ struct cpab_ctx {
   size_t items_cap = 0;
   unsigned int* queue = nullptr;
   std::vector<WorkItem> h_items;
+  unsigned char* static_scratch = nullptr;  // this device's copy of `scratch` (cpab_calc_T_items_noalloc)
 };
+
+// Scratch of cpab_calc_T_items_noalloc, for callers that cannot allocate:
+// the launch's queue counter (the first 256 bytes) and up to
+// kStaticItems work items.  One copy per GPU, held until the process ends;
+// every call on a GPU shares it.  Host code never passes this name to a
+// kernel: it is a host-side stand-in; the device address comes from
+// hipGetSymbolAddress (cpab_create, once per device).
+constexpr int kStaticItems = 4096;
+constexpr size_t kStaticQueueBytes = 256;
+constexpr size_t kStaticScratchBytes = kStaticQueueBytes + (size_t)kStaticItems * sizeof(WorkItem);
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kMaxDevices] = {};
 
 namespace {
 
@@ -300,6 +316,18 @@ int cpab_create(int device, cpab_ctx** out) {
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
     ctx->max_lds = v;
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0) ctx->n_cu = v;
+  if (device < kMaxDevices) {  // the device address of this GPU's `scratch`, looked up once per device
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+      void* p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        delete ctx;
+        return CPAB_ERR_HIP;
+      }
+      g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    }
+    ctx->static_scratch = g_scratch_ptr[device];
+  }
   *out = ctx;
   return CPAB_OK;
 }
@@ -994,7 +1022,8 @@ int cpab_batch_destroy(cpab_batch* b) {
 namespace {
 
 template <int DIM, int TESS, bool POW2>
-int items_launch(cpab_ctx* ctx, int nItems, int maxNC, const cpab_batch_sched& q, double h, int nT, int nS) {
+int items_launch(cpab_ctx* ctx, const WorkItem* items, unsigned int* queue, int nItems, int maxNC,
+                 const cpab_batch_sched& q, double h, int nT, int nS) {
   constexpr int NE = DIM * (DIM + 1);
   const size_t lds = (size_t)2 * maxNC * NE * sizeof(double);
   auto k = k_integrate_items<DIM, TESS, POW2>;
@@ -1005,9 +1034,8 @@ int items_launch(cpab_ctx* ctx, int nItems, int maxNC, const cpab_batch_sched& q
   int grid = ctx->n_cu * per_cu;
   const int64_t chunks = ((int64_t)nItems + q.chunk - 1) / q.chunk;
   if (grid > chunks) grid = (int)chunks;
-  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->queue);
-  hipLaunchKernelGGL(k, dim3(grid), dim3(q.block), lds, ctx->stream, ctx->items, nItems, q.chunk, ctx->queue, h, nT,
-                     nS);
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, ctx->stream, queue);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(q.block), lds, ctx->stream, items, nItems, q.chunk, queue, h, nT, nS);
   HIP_OK(ctx, hipGetLastError());
   return CPAB_OK;
 }
@@ -1028,49 +1056,104 @@ int cpab_theta2avees(cpab_ctx* ctx, int D, int d, int m, const double* B, const 
   return CPAB_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Checks of a work list and the device form of item i (no allocation).
+int item_convert(cpab_ctx* ctx, int dim, int tess, const cpab_work_item& q2, int i, WorkItem* W, bool* pow2) {
+  std::memset(W, 0, sizeof *W);
+  int r = make_geom(ctx, dim, tess, q2.nC0, q2.nC1, q2.nC2, q2.inc_x, q2.inc_y, q2.inc_z, &W->g, pow2);
+  if (r) return r;
+  int64_t boxes = q2.nC0;
+  if (dim >= 2) boxes *= q2.nC1;
+  if (dim >= 3) boxes *= q2.nC2;
+  const int64_t nC = boxes * (tess == TESS_II ? 1 : (dim == 2 ? 4 : 5));
+  if ((r = check_nC(ctx, dim, tess, (int)nC, q2.nC0, q2.nC1, q2.nC2))) return r;
+  if (!q2.trels || !q2.As || !q2.pos0 || !q2.out || q2.begin < 0 || q2.end < q2.begin)
+    return fail(ctx, CPAB_ERR_ARG, "item %d: null pointer or bad range", i);
+  W->nC = (int)nC;
+  W->trels = q2.trels;
+  W->as = q2.As;
+  W->pos0 = q2.pos0;
+  W->out = q2.out;
+  W->begin = q2.begin;
+  W->end = q2.end;
+  return CPAB_OK;
+}
+
+// Argument checks shared by the item entries; the schedule in *q, the
+// largest tessellation and the increment kind of the list.
+int items_check(cpab_ctx* ctx, int dim, int tess, int nItems, const cpab_work_item* items, int nTimeSteps,
+                int nStepsODE, const cpab_batch_sched* sched, cpab_batch_sched* q, int* maxNC, bool* pow2_all) {
+  if (nItems < 0 || (nItems > 0 && !items)) return fail(ctx, CPAB_ERR_ARG, "bad item list");
+  if (nTimeSteps < 0 || nStepsODE < 0) return fail(ctx, CPAB_ERR_ARG, "negative step counts");
+  *q = sched ? *sched : cpab_batch_sched{16384, 1024, 1};
+  if (q->block < 64 || q->block > 1024 || (q->block & 63) || q->chunk < 1)
+    return fail(ctx, CPAB_ERR_ARG, "schedule: block a multiple of 64 up to 1024, chunk >= 1");
+  bool all = true, any = false;
+  int mx = 0;
+  for (int i = 0; i < nItems; i++) {
+    WorkItem W;
+    bool pow2;
+    int r = item_convert(ctx, dim, tess, items[i], i, &W, &pow2);
+    if (r) return r;
+    all = all && pow2;
+    any = any || pow2;
+    if (W.nC > mx) mx = W.nC;
+  }
+  if (any && !all)
+    return fail(ctx, CPAB_ERR_UNSUPPORTED, "items mix power-of-two and other increments: launch each kind apart");
+  const int NE = dim * (dim + 1);
+  if ((size_t)2 * mx * NE * sizeof(double) > (size_t)ctx->max_lds)
+    return fail(ctx, CPAB_ERR_UNSUPPORTED, "the largest tessellation's tables exceed LDS");
+  *maxNC = mx;
+  *pow2_all = all;
+  return CPAB_OK;
+}
+
+int items_dispatch(cpab_ctx* ctx, const WorkItem* d_items, unsigned int* queue, int dim, int tess, bool pow2_all,
+                   int nItems, int maxNC, const cpab_batch_sched& q, double dt, int nTimeSteps, int nStepsODE) {
+  const double h = dtdiv(dt, nStepsODE);
+  int r = CPAB_OK;
+#define ITEMS_CASE(D_, T_, P_)                                                                                    \
+  if (dim == D_ && tess == T_ && pow2_all == P_)                                                                 \
+    r = items_launch<D_, T_, P_>(ctx, d_items, queue, nItems, maxNC, q, h, nTimeSteps, nStepsODE);               \
+  else
+  ITEMS_CASE(1, TESS_II, true)
+  ITEMS_CASE(1, TESS_II, false)
+  ITEMS_CASE(2, TESS_I, true)
+  ITEMS_CASE(2, TESS_I, false)
+  ITEMS_CASE(2, TESS_II, true)
+  ITEMS_CASE(2, TESS_II, false)
+  ITEMS_CASE(3, TESS_I, true)
+  ITEMS_CASE(3, TESS_I, false)
+  ITEMS_CASE(3, TESS_II, true)
+  ITEMS_CASE(3, TESS_II, false)
+  r = fail(ctx, CPAB_ERR_UNSUPPORTED, "no item kernel for dim %d tess %d", dim, tess);
+#undef ITEMS_CASE
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
 int cpab_calc_T_items(cpab_ctx* ctx, int dim, int tess, int nItems, const cpab_work_item* items, double dt,
                       int nTimeSteps, int nStepsODE, const cpab_batch_sched* sched) {
   if (!ctx) return CPAB_ERR_ARG;
-  if (nItems < 0 || (nItems > 0 && !items)) return fail(ctx, CPAB_ERR_ARG, "bad item list");
+  cpab_batch_sched q;
+  int maxNC = 0;
+  bool pow2_all = true;
+  int r = items_check(ctx, dim, tess, nItems, items, nTimeSteps, nStepsODE, sched, &q, &maxNC, &pow2_all);
+  if (r) return r;
   if (nItems == 0) return CPAB_OK;
-  if (nTimeSteps < 0 || nStepsODE < 0) return fail(ctx, CPAB_ERR_ARG, "negative step counts");
-  cpab_batch_sched q = sched ? *sched : cpab_batch_sched{16384, 1024, 1};
-  if (q.block < 64 || q.block > 1024 || (q.block & 63) || q.chunk < 1)
-    return fail(ctx, CPAB_ERR_ARG, "schedule: block a multiple of 64 up to 1024, chunk >= 1");
-  const int NE = dim * (dim + 1);
   std::vector<WorkItem>& hv = ctx->h_items;
   hv.resize(nItems);
-  int maxNC = 0;
-  bool pow2_all = true, pow2_any = false;
   for (int i = 0; i < nItems; i++) {
-    const cpab_work_item& q2 = items[i];
-    WorkItem& W = hv[i];
-    std::memset(&W, 0, sizeof W);
     bool pow2;
-    int r = make_geom(ctx, dim, tess, q2.nC0, q2.nC1, q2.nC2, q2.inc_x, q2.inc_y, q2.inc_z, &W.g, &pow2);
-    if (r) return r;
-    int64_t boxes = q2.nC0;
-    if (dim >= 2) boxes *= q2.nC1;
-    if (dim >= 3) boxes *= q2.nC2;
-    const int64_t nC = boxes * (tess == TESS_II ? 1 : (dim == 2 ? 4 : 5));
-    if ((r = check_nC(ctx, dim, tess, (int)nC, q2.nC0, q2.nC1, q2.nC2))) return r;
-    if (!q2.trels || !q2.As || !q2.pos0 || !q2.out || q2.begin < 0 || q2.end < q2.begin)
-      return fail(ctx, CPAB_ERR_ARG, "item %d: null pointer or bad range", i);
-    pow2_all = pow2_all && pow2;
-    pow2_any = pow2_any || pow2;
-    W.nC = (int)nC;
-    W.trels = q2.trels;
-    W.as = q2.As;
-    W.pos0 = q2.pos0;
-    W.out = q2.out;
-    W.begin = q2.begin;
-    W.end = q2.end;
-    if (nC > maxNC) maxNC = (int)nC;
+    if ((r = item_convert(ctx, dim, tess, items[i], i, &hv[i], &pow2))) return r;
   }
-  if (pow2_any && !pow2_all)
-    return fail(ctx, CPAB_ERR_UNSUPPORTED, "items mix power-of-two and other increments: launch each kind apart");
-  if ((size_t)2 * maxNC * NE * sizeof(double) > (size_t)ctx->max_lds)
-    return fail(ctx, CPAB_ERR_UNSUPPORTED, "the largest tessellation's tables exceed LDS");
   if (!ctx->queue) HIP_OK(ctx, hipMalloc(&ctx->queue, 256));
   if (ctx->items_cap < (size_t)nItems) {
     HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1085,26 +1168,43 @@ int cpab_calc_T_items(cpab_ctx* ctx, int dim, int tess, int nItems, const cpab_w
   // vector can be refilled by the next call; stream order protects the device
   // buffer from a launch still reading it
   HIP_OK(ctx, hipMemcpyAsync(ctx->items, hv.data(), sizeof(WorkItem) * nItems, hipMemcpyHostToDevice, ctx->stream));
-  const double h = dtdiv(dt, nStepsODE);
   hipEvent_t a = nullptr, e = nullptr;
-  int r = prof_begin(ctx, &a, &e);
-  if (r) return r;
-#define ITEMS_CASE(D_, T_, P_) \
-  if (dim == D_ && tess == T_ && pow2_all == P_) r = items_launch<D_, T_, P_>(ctx, nItems, maxNC, q, h, nTimeSteps, nStepsODE); else
-  ITEMS_CASE(1, TESS_II, true)
-  ITEMS_CASE(1, TESS_II, false)
-  ITEMS_CASE(2, TESS_I, true)
-  ITEMS_CASE(2, TESS_I, false)
-  ITEMS_CASE(2, TESS_II, true)
-  ITEMS_CASE(2, TESS_II, false)
-  ITEMS_CASE(3, TESS_I, true)
-  ITEMS_CASE(3, TESS_I, false)
-  ITEMS_CASE(3, TESS_II, true)
-  ITEMS_CASE(3, TESS_II, false)
-  r = fail(ctx, CPAB_ERR_UNSUPPORTED, "no item kernel for dim %d tess %d", dim, tess);
-#undef ITEMS_CASE
+  if ((r = prof_begin(ctx, &a, &e))) return r;
+  r = items_dispatch(ctx, ctx->items, ctx->queue, dim, tess, pow2_all, nItems, maxNC, q, dt, nTimeSteps, nStepsODE);
   if (r) return r;
   return prof_end(ctx, a, e);
+}
+
+int cpab_calc_T_items_noalloc(cpab_ctx* ctx, int dim, int tess, int nItems, const cpab_work_item* items, double dt,
+                              int nTimeSteps, int nStepsODE, const cpab_batch_sched* sched) {
+  if (!ctx) return CPAB_ERR_ARG;
+  if (nItems > kStaticItems)
+    return fail(ctx, CPAB_ERR_ARG, "%d work items: the allocation-free entry takes at most %d per call", nItems,
+                kStaticItems);
+  if (!ctx->static_scratch)
+    return fail(ctx, CPAB_ERR_UNSUPPORTED, "no static scratch on device %d (devices 0..%d have one)", ctx->device,
+                kMaxDevices - 1);
+  cpab_batch_sched q;
+  int maxNC = 0;
+  bool pow2_all = true;
+  int r = items_check(ctx, dim, tess, nItems, items, nTimeSteps, nStepsODE, sched, &q, &maxNC, &pow2_all);
+  if (r) return r;
+  if (nItems == 0) return CPAB_OK;
+  unsigned int* queue = reinterpret_cast<unsigned int*>(ctx->static_scratch);
+  WorkItem* d_items = reinterpret_cast<WorkItem*>(ctx->static_scratch + kStaticQueueBytes);
+  // the device form of the list, converted in stack-sized chunks; each
+  // pageable copy is staged before it returns, so the chunk can be refilled
+  constexpr int kChunk = 64;
+  WorkItem buf[kChunk];
+  for (int i0 = 0; i0 < nItems; i0 += kChunk) {
+    const int n = nItems - i0 < kChunk ? nItems - i0 : kChunk;
+    for (int i = 0; i < n; i++) {
+      bool pow2;
+      if ((r = item_convert(ctx, dim, tess, items[i0 + i], i0 + i, &buf[i], &pow2))) return r;
+    }
+    HIP_OK(ctx, hipMemcpyAsync(d_items + i0, buf, sizeof(WorkItem) * n, hipMemcpyHostToDevice, ctx->stream));
+  }
+  return items_dispatch(ctx, d_items, queue, dim, tess, pow2_all, nItems, maxNC, q, dt, nTimeSteps, nStepsODE);
 }
 
 // ------------------------------------------------------------- MCMC

@@ -254,6 +254,19 @@ int cpab_theta2avees(cpab_ctx *ctx, int D, int d, int m, const double *B, const 
 int cpab_calc_T_items(cpab_ctx *ctx, int dim, int tess, int nItems, const cpab_work_item *items,
                       double dt, int nTimeSteps, int nStepsODE, const cpab_batch_sched *sched);
 
+/* cpab_calc_T_items without any allocation: no reference counterpart; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * kernels and results as cpab_calc_T_items (bit for bit); the work list and
+ * the launch's queue counter live in a static device array of the library
+ * (one per GPU, 640 KiB, held until the process ends), so a call takes
+ * at most 4096 items (CPAB_ERR_ARG beyond).  Every call on a GPU shares that
+ * array: calls that overlap on one GPU (other streams or contexts) race; calls
+ * on one stream are ordered.  No kernel timing (cpab_prof_*) for this entry. */
+int cpab_calc_T_items_noalloc(cpab_ctx *ctx, int dim, int tess, int nItems,
+                              const cpab_work_item *items, double dt, int nTimeSteps,
+                              int nStepsODE, const cpab_batch_sched *sched);
+
 /* ------------------------------------------------------------------------
  * Device-resident Metropolis sampling of 2D image registration, for many
  * independent chains at once: Metropolis.run / .step
