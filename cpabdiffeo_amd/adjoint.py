@@ -323,14 +323,48 @@ class PairsProblem:
     def loss(self, theta):
         return warp_ssd(theta, self.Bt, self.src, self.dst, self.space, self.params, sign=-1, simple=self.simple)
 
-    def loss_and_signature(self, theta):
-        """(loss (nInst,), per-pixel lookup signature (nInst, n*n) int32) of the
-        calc_T map, without the gradient's autograd tape."""
-        torch = require_gpu()
-        As, T = _tables(torch, theta.detach(), self.Bt, -1, self.space.nC, self.params["dt"])
-        sig = torch.empty((self.nInst, self.n * self.n), dtype=torch.int32, device="cuda")
-        loss, _ = ssd_loss_and_grad_T(self.space, As, T, self.src, self.dst, self.params, sig=sig)
-        return loss, sig
+
+def fd_check_same_lookups(problem, theta, v, h=1e-6):
+    """Central differences of the calc_T-map SSD loss along v (nInst, d),
+    restricted per pair to the pixels whose cell lookups are the same at
+    theta + h v and theta - h v (there the map is smooth between the two; a
+    pixel whose accept/redo decision or substep cell changes jumps by the
+    difference of a Trel step and its RK2 redo).  Returns, per pair, (the
+    central difference over those pixels, the adjoint's directional
+    derivative of the loss over the same pixels -- cpab_calc_T_vjp with the
+    other pixels' cotangents zeroed --, the count of pixels left out)."""
+    torch = require_gpu()
+    sp, B, p = problem.space, problem.Bt, problem.params
+    n, nI = problem.n, problem.nInst
+    N = n * n
+
+    def ends(th):
+        As, T = _tables(torch, th, B, -1, sp.nC, p["dt"])
+        pos = torch.empty((nI, N, 2), dtype=torch.float64, device="cuda")
+        sig = torch.empty((nI, N), dtype=torch.int32, device="cuda")
+        ssd_loss_and_grad_T(sp, As, T, problem.src, problem.dst, p, pos_out=pos, sig=sig)
+        return As, T, pos, sig
+
+    _, _, pp, sgp = ends(theta + h * v)
+    _, _, pm, sgm = ends(theta - h * v)
+    As, T, pos, _ = ends(theta)
+    same = sgp == sgm
+    fd = torch.empty(nI, dtype=torch.float64, device="cuda")
+    lam = torch.empty((nI, N, 2), dtype=torch.float64, device="cuda")
+    for i in range(nI):
+        wp, _ = ssd_cotangent(problem.src[i], problem.dst[i], pp[i])
+        wm, _ = ssd_cotangent(problem.src[i], problem.dst[i], pm[i])
+        d = problem.dst[i].reshape(-1).double()
+        lp = 0.5 * (wp - d) ** 2
+        lm = 0.5 * (wm - d) ** 2
+        fd[i] = ((lp - lm) * same[i]).sum() / (2 * h)
+        _, l0 = ssd_cotangent(problem.src[i], problem.dst[i], pos[i])
+        lam[i] = l0 * same[i].unsqueeze(1)
+    grid = sp.tessellation.dense_grid_img()
+    pts = torch.from_numpy(grid).cuda().unsqueeze(0).expand(nI, N, 2).contiguous()
+    gA = calc_T_vjp(sp, pts, lam, As, T, p)
+    an = -((gA.reshape(nI, -1) @ B) * v).sum(1)
+    return fd, an, (~same).sum(1)
 
 
 def gradient_descent(problem, steps, lr, theta0=None):

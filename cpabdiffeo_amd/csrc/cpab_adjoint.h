@@ -26,13 +26,14 @@
 // sensitivity solveODE2, calc_2Dto2D.cu:202-319).
 //
 // Schedule, one lane per point: the forward sweep writes the point at the
-// start of every time step to a checkpoint buffer (nT points per point); the
-// backward sweep walks the time steps in reverse, recomputes the step's
-// decision from its checkpoint and, for a redone step, its nS substep start
-// points into registers (ADJ_NSMAX of them, unrolled so they stay in
-// registers), then runs the step's adjoint units in reverse: a redone step's
-// substeps j = nS-1 .. 0, an accepted step's Trel step at j = 0 (a wave
-// whose lanes all accepted runs only that unit).
+// start of every time step to a checkpoint buffer (nT points per point) and,
+// for calc_T, one bit per step recording whether the step was redone; the
+// backward sweep walks the time steps in reverse (the next checkpoint loaded
+// while a step is processed), recomputes a redone step's nS substep start
+// points from its checkpoint into registers (NSM of them, unrolled so they
+// stay in registers), then runs the step's adjoint units in reverse: a
+// redone step's substeps j = nS-1 .. 0, an accepted step's Trel step at
+// j = 0 (a wave whose lanes all accepted runs only that unit).
 //
 // Accumulation, bit-reproducible: each lane keeps two register
 // accumulators, dJ/dA of its current RK2 cell and dJ/dTrel of its current
@@ -55,7 +56,7 @@
 
 namespace cpab {
 
-constexpr int ADJ_NSMAX = 16;  // substeps per time step kept in registers
+constexpr int ADJ_NSMAX = 16;  // substeps per time step kept in registers (the kernel's NSM is 10 or 16)
 
 struct AdjDesc {
   int nInst, nC;
@@ -70,6 +71,8 @@ struct AdjDesc {
   const float* dst;      // MODE 1: (nInst, H, W)
   uint32_t* sig;         // optional (nInst, nPts): hash of every cell the forward sweep looked up
   double* ckpt;          // (nInst, nT, nPts, 2) workspace
+  uint64_t* redo_bits;   // calc_T map: (nInst, ceil(nT / 64), nPts) workspace, bit k % 64 of word k / 64:
+                         // step k was redone by RK2
   double* partial;       // (nInst, blocks per instance, NG*nC*6 + 2): tables, loss, redone steps
   double h;
   int nT, nS;
@@ -143,8 +146,8 @@ __host__ __device__ inline size_t adj_lds_doubles(int nC, bool fwd, int waves) {
   return NG * NT + (size_t)waves * NG * NT;
 }
 
-template <int TESS, bool POW2, int MODE, bool FWD>
-__global__ void __launch_bounds__(256) k_adjoint2d(AdjDesc D) {
+template <int TESS, bool POW2, int MODE, bool FWD, int NSM>
+__global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
   extern __shared__ double lds[];
   __shared__ double red[4];
   constexpr int NG = FWD ? 2 : 1;
@@ -180,26 +183,38 @@ __global__ void __launch_bounds__(256) k_adjoint2d(AdjDesc D) {
       p[0] = (double)(idx % D.W);
       p[1] = (double)(idx / D.W);
     }
-    // forward sweep, one checkpoint per time step
+    // forward sweep, one checkpoint per time step (and, for calc_T, the
+    // step's decision, 64 steps to a word)
     double* ck = D.ckpt + (int64_t)inst * nT * nPts * 2;
+    const int nWords = (nT + 63) >> 6;
+    uint64_t* rb = FWD ? D.redo_bits + (int64_t)inst * nWords * nPts : nullptr;
     uint32_t sg = 0;
+    uint64_t bits = 0;
     for (int k = 0; k < nT; k++) {
       ck[((int64_t)k * nPts + idx) * 2] = p[0];
       ck[((int64_t)k * nPts + idx) * 2 + 1] = p[1];
+      bool redo = true;
       if (FWD) {
         const int c = cell2<TESS, POW2>(p, g);
         double pn[2];
         affine<2>(pn, Tr + c * 6, p);
         const int cn = cell2<TESS, POW2>(pn, g);
         sg = sig_mix(sig_mix(sg, c), cn);
-        if (cn == c) {
+        redo = cn != c;
+        if (!redo) {
           p[0] = pn[0];
           p[1] = pn[1];
-          continue;
+        }
+        bits |= (uint64_t)redo << (k & 63);
+        if ((k & 63) == 63 || k == nT - 1) {
+          rb[(int64_t)(k >> 6) * nPts + idx] = bits;
+          bits = 0;
         }
       }
-      nredo++;
-      for (int j = 0; j < nS; j++) sg = sig_mix(sg, rk2_substep<TESS, POW2>(p, At, h, g));
+      if (redo) {
+        nredo++;
+        for (int j = 0; j < nS; j++) sg = sig_mix(sg, rk2_substep<TESS, POW2>(p, At, h, g));
+      }
     }
     if (D.sig) D.sig[ib + idx] = sg;
     if (D.pos_out) {
@@ -223,22 +238,27 @@ __global__ void __launch_bounds__(256) k_adjoint2d(AdjDesc D) {
     int ca = -1, ct = -1;  // cells of the RK2 and Trel accumulators (-1: empty)
     double Ga[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double Gt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    // the checkpoint of the step before is loaded while a step is processed
+    double qn[2] = {ck[((int64_t)(nT - 1) * nPts + idx) * 2], ck[((int64_t)(nT - 1) * nPts + idx) * 2 + 1]};
     for (int k = nT - 1; k >= 0; k--) {
-      const double q[2] = {ck[((int64_t)k * nPts + idx) * 2], ck[((int64_t)k * nPts + idx) * 2 + 1]};
+      const double q[2] = {qn[0], qn[1]};
+      if (k > 0) {
+        qn[0] = ck[((int64_t)(k - 1) * nPts + idx) * 2];
+        qn[1] = ck[((int64_t)(k - 1) * nPts + idx) * 2 + 1];
+      }
       bool redo = true;
       int c0 = 0;
       if (FWD) {
-        c0 = cell2<TESS, POW2>(q, g);
-        double pn[2];
-        affine<2>(pn, Tr + c0 * 6, q);
-        redo = cell2<TESS, POW2>(pn, g) != c0;
+        if (k == nT - 1 || (k & 63) == 63) bits = rb[(int64_t)(k >> 6) * nPts + idx];
+        redo = (bits >> (k & 63)) & 1;
+        if (!redo) c0 = cell2<TESS, POW2>(q, g);
       }
       const bool any_redo = FWD ? (__ballot(redo) != 0) : true;  // wave-uniform
-      double ps[ADJ_NSMAX][2];
+      double ps[NSM][2];
       if (any_redo) {
         double r[2] = {q[0], q[1]};
 #pragma unroll
-        for (int j = 0; j < ADJ_NSMAX; j++) {
+        for (int j = 0; j < NSM; j++) {
           if (j < nS) {
             ps[j][0] = r[0];
             ps[j][1] = r[1];
@@ -247,7 +267,7 @@ __global__ void __launch_bounds__(256) k_adjoint2d(AdjDesc D) {
         }
       }
 #pragma unroll
-      for (int j = ADJ_NSMAX - 1; j >= 0; j--) {
+      for (int j = NSM - 1; j >= 0; j--) {
         if (j < nS && (any_redo || j == 0)) {  // wave-uniform
           if (redo) {  // substep j of a redone step
             const double* pj = ps[j];

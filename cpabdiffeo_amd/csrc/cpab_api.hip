@@ -658,7 +658,8 @@ int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts
   if (nbx > 0x7fffffffLL) return fail(ctx, CPAB_ERR_ARG, "too many points");
   const size_t ck = (size_t)nInst * nT * nPts * 2, pa = (size_t)nInst * nbx * (NG * NT + 2);
   const size_t gt = fwd ? (size_t)nInst * NT : 0;
-  if ((r = ensure_ws(ctx, (ck + pa + gt) * sizeof(double)))) return r;
+  const size_t rb = fwd ? (size_t)nInst * ((nT + 63) / 64) * nPts : 0;  // 64-bit words
+  if ((r = ensure_ws(ctx, (ck + pa + gt + rb) * sizeof(double)))) return r;
   AdjDesc D{};
   D.nInst = nInst;
   D.nC = nC;
@@ -675,6 +676,8 @@ int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts
   D.sig = sig;
   D.ckpt = ctx->ws;
   D.partial = ctx->ws + ck;
+  D.redo_bits = fwd ? reinterpret_cast<uint64_t*>(ctx->ws + ck + pa + gt) : nullptr;
+  const bool ns10 = nS <= 10;  // the substep starts of a redone step in 10 registers pairs, else 16
   D.h = dt / (double)nS;
   D.nT = nT;
   D.nS = nS;
@@ -682,17 +685,22 @@ int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts
   const dim3 grid((unsigned)nbx, (unsigned)nInst);
   hipEvent_t a = nullptr, b = nullptr;
   if ((r = prof_begin(ctx, &a, &b))) return r;
-#define ADJ_LAUNCH3(T_, P_, M_)                                             \
+#define ADJ_LAUNCH4(T_, P_, M_, F_)                                        \
   do {                                                                     \
-    if (fwd) {                                                             \
-      auto k = k_adjoint2d<T_, P_, M_, true>;                              \
+    if (ns10) {                                                            \
+      auto k = k_adjoint2d<T_, P_, M_, F_, 10>;                            \
       allow_lds(k, lds);                                                   \
       hipLaunchKernelGGL(k, grid, dim3(block), lds, ctx->stream, D);       \
     } else {                                                               \
-      auto k = k_adjoint2d<T_, P_, M_, false>;                             \
+      auto k = k_adjoint2d<T_, P_, M_, F_, ADJ_NSMAX>;                     \
       allow_lds(k, lds);                                                   \
       hipLaunchKernelGGL(k, grid, dim3(block), lds, ctx->stream, D);       \
     }                                                                      \
+  } while (0)
+#define ADJ_LAUNCH3(T_, P_, M_)                                            \
+  do {                                                                     \
+    if (fwd) ADJ_LAUNCH4(T_, P_, M_, true);                                \
+    else ADJ_LAUNCH4(T_, P_, M_, false);                                   \
   } while (0)
 #define ADJ_LAUNCH(T_, P_)                   \
   do {                                       \
@@ -708,6 +716,7 @@ int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts
   }
 #undef ADJ_LAUNCH
 #undef ADJ_LAUNCH3
+#undef ADJ_LAUNCH4
   HIP_OK(ctx, hipGetLastError());
   if ((r = prof_end(ctx, a, b))) return r;
   double* gradT = fwd ? ctx->ws + ck + pa : nullptr;

@@ -353,3 +353,21 @@ def test_simple_adjoint_is_bit_reproducible(pix, orc):
     As = t((-(0.5 * star @ B.T)).reshape(3, sp.nC, 6))
     r = [adjoint.ssd_loss_and_grad(sp, As, t(src), t(dst), params) for _ in range(3)]
     assert all(torch.equal(r[0][1], x[1]) and torch.equal(r[0][0], x[0]) for x in r[1:])
+
+
+def test_fd_check_over_same_lookup_pixels(gpu):
+    """The full-size gradient check's procedure at test size: central
+    differences over the pixels whose lookups are the same at theta +- h v
+    agree with the adjoint over the same pixels to 1e-4 on every pair."""
+    import torch
+    from cpabdiffeo_amd import adjoint
+    say("FD over same-lookup pixels")
+    prob = adjoint.PairsProblem(n=32, nInst=6, seed=2, nTimeSteps=50)
+    rng = np.random.default_rng(4)
+    theta = torch.from_numpy(0.5 * prob.theta_star).cuda()
+    v = torch.from_numpy(rng.standard_normal(theta.shape)).cuda()
+    fd, an, left = adjoint.fd_check_same_lookups(prob, theta, v)
+    rel = ((fd - an).abs() / an.abs()).cpu().numpy()
+    say("rel", rel, "pixels left out", left.cpu().numpy())
+    assert (rel < 1e-4).all()
+    assert int(left.max()) < prob.n * prob.n // 4
