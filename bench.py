@@ -222,6 +222,13 @@ def config3_extra(torch, ctx):
     plan.calc_T(tables, pts, out, params["dt"], 100, 10, use_graph=True)
     cnt, ms = ctx.prof_read()
     ctx.prof_enable(False)
+    # rejected Trel proposals (each redone by 10 RK2 substeps), counted by the
+    # same kernel in a launch of its own: the algorithmic work of the batch
+    nfb = torch.zeros(plan.nPts, dtype=torch.int32, device="cuda")
+    plan.calc_T(tables, pts, out, params["dt"], 100, 10, use_graph=False, n_fallback=nfb)
+    fb = int(nfb.sum(dtype=torch.int64))
+    del nfb
+    bflops = plan.nPts * 100 * F_FAST + fb * 10 * F_SUB
     out2 = torch.empty_like(pts)
     batch.plain_per_instance(spaces, insts, pts, out2, params)
     t_p = batch.timed(lambda: batch.plain_per_instance(spaces, insts, pts, out2, params))
@@ -229,7 +236,8 @@ def config3_extra(torch, ctx):
     a = {"workload": "1024 instances, tess {4,8,16}^2 x 4 triangles, N ~ logU[1e3,1e6] (total %d points), "
                      "calc_T_fwd t=1 100x10 fp64" % plan.nPts,
          "points": plan.nPts, "batched_ms": t_b * 1e3, "batched_points_per_sec": plan.nPts / t_b,
-         "batched_kernel_ms": ms / cnt, "schedule": list(plan.schedule.key()),
+         "batched_kernel_ms": ms / cnt, "fallback_steps": fb, "alg_tflops": bflops / (ms / cnt * 1e-3) / 1e12,
+         "frac_fp64_peak": bflops / (ms / cnt * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "schedule": list(plan.schedule.key()),
          "autotune_ms": {"%d/%d/%d" % k: v for k, v in tune.items()},
          "plain_ms": t_p * 1e3, "plain_points_per_sec": plan.nPts / t_p, "speedup": t_p / t_b,
          "bit_identical_to_plain": same}
@@ -318,10 +326,11 @@ def config4_extra(torch, ctx, steps=200, lr=3e-4):
     kus = ms / cnt * 1e3
     # central differences (h = 1e-6, FP64 loss) along one random direction per
     # pair: (1) of the whole loss against the adjoint; (2) over the pixels whose
-    # cell lookups are the same at theta + h v and theta - h v (a pixel whose
-    # accept/redo decision or substep cell changes jumps by the difference of a
-    # Trel step and its RK2 redo: not a derivative), against the adjoint of the
-    # loss over the same pixels; the pixels left out are counted
+    # cell lookups and bilinear sample cell are the same at theta + h v and
+    # theta - h v (a pixel whose accept/redo decision or substep cell changes
+    # jumps by the difference of a Trel step and its RK2 redo: not a
+    # derivative), against the adjoint of the loss over the same pixels; the
+    # pixels left out are counted
     rng = np.random.default_rng(0)
     th = theta.clone().requires_grad_(True)
     loss = prob.loss(th)

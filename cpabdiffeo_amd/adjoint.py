@@ -326,10 +326,12 @@ class PairsProblem:
 
 def fd_check_same_lookups(problem, theta, v, h=1e-6):
     """Central differences of the calc_T-map SSD loss along v (nInst, d),
-    restricted per pair to the pixels whose cell lookups are the same at
-    theta + h v and theta - h v (there the map is smooth between the two; a
-    pixel whose accept/redo decision or substep cell changes jumps by the
-    difference of a Trel step and its RK2 redo).  Returns, per pair, (the
+    restricted per pair to the pixels whose cell lookups, and the pixel cell
+    of whose bilinear sample, are the same at theta + h v and theta - h v
+    (there the loss is smooth between the two; a pixel whose accept/redo
+    decision or substep cell changes jumps by the difference of a Trel step
+    and its RK2 redo, and the interpolant's gradient jumps across pixel
+    lines).  Returns, per pair, (the
     central difference over those pixels, the adjoint's directional
     derivative of the loss over the same pixels -- cpab_calc_T_vjp with the
     other pixels' cotangents zeroed --, the count of pixels left out)."""
@@ -348,7 +350,9 @@ def fd_check_same_lookups(problem, theta, v, h=1e-6):
     _, _, pp, sgp = ends(theta + h * v)
     _, _, pm, sgm = ends(theta - h * v)
     As, T, pos, _ = ends(theta)
-    same = sgp == sgm
+    # the same lookups, and the same bilinear cell of the sample (the
+    # interpolant's gradient jumps across pixel lines)
+    same = (sgp == sgm) & (torch.floor(pp) == torch.floor(pm)).all(dim=2)
     fd = torch.empty(nI, dtype=torch.float64, device="cuda")
     lam = torch.empty((nI, N, 2), dtype=torch.float64, device="cuda")
     for i in range(nI):

@@ -76,6 +76,7 @@ struct AdjDesc {
   double* partial;       // (nInst, blocks per instance, NG*nC*6 + 2): tables, loss, redone steps
   double h;
   int nT, nS;
+  int tile8;             // MODE 1: a wave's 64 pixels are an 8 x 8 tile (W and H multiples of 8)
   Geom g;
 };
 
@@ -175,13 +176,23 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
   int nredo = 0;  // time steps redone by RK2 (every step of calc_T_simple)
   if (idx < nPts) {
     const int64_t ib = (int64_t)inst * nPts;
+    // the point's index in the caller's arrays: idx, or in MODE 1 with tiles
+    // the pixel of lane idx & 63 in 8 x 8 tile idx >> 6 (neighbouring pixels
+    // cross facets at about the same steps, so fewer of a wave's steps have
+    // a redone lane; the checkpoints stay indexed by idx)
+    int64_t pix = idx;
+    if (MODE == 1 && D.tile8) {
+      const int64_t t = idx >> 6;
+      const int u = (int)(idx & 63), tw = D.W >> 3;
+      pix = ((t / tw) * 8 + (u >> 3)) * (int64_t)D.W + (t % tw) * 8 + (u & 7);
+    }
     double p[2];
     if (MODE == 0) {
       p[0] = D.pos0[(ib + idx) * 2];
       p[1] = D.pos0[(ib + idx) * 2 + 1];
     } else {
-      p[0] = (double)(idx % D.W);
-      p[1] = (double)(idx / D.W);
+      p[0] = (double)(pix % D.W);
+      p[1] = (double)(pix / D.W);
     }
     // forward sweep, one checkpoint per time step (and, for calc_T, the
     // step's decision, 64 steps to a word)
@@ -216,10 +227,10 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
         for (int j = 0; j < nS; j++) sg = sig_mix(sg, rk2_substep<TESS, POW2>(p, At, h, g));
       }
     }
-    if (D.sig) D.sig[ib + idx] = sg;
+    if (D.sig) D.sig[ib + pix] = sg;
     if (D.pos_out) {
-      D.pos_out[(ib + idx) * 2] = p[0];
-      D.pos_out[(ib + idx) * 2 + 1] = p[1];
+      D.pos_out[(ib + pix) * 2] = p[0];
+      D.pos_out[(ib + pix) * 2 + 1] = p[1];
     }
     double lam[2];
     if (MODE == 0) {
@@ -229,7 +240,7 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
       const int64_t io = (int64_t)inst * D.H * D.W;
       double gx, gy;
       const double w = sample2_grad(D.src + io, D.H, D.W, p[0], p[1], &gx, &gy);
-      const double r = w - (double)D.dst[io + idx];
+      const double r = w - (double)D.dst[io + pix];
       loss = 0.5 * r * r;
       lam[0] = r * gx;
       lam[1] = r * gy;

@@ -681,6 +681,7 @@ int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts
   D.h = dt / (double)nS;
   D.nT = nT;
   D.nS = nS;
+  D.tile8 = (mode == 1 && H % 8 == 0 && W % 8 == 0) ? 1 : 0;
   D.g = g;
   const dim3 grid((unsigned)nbx, (unsigned)nInst);
   hipEvent_t a = nullptr, b = nullptr;
@@ -838,6 +839,10 @@ int batch_launch(cpab_batch* b, hipStream_t s, const cpab_batch_sched& q, const 
                  double* out, double h, int nT, int nS, int32_t* nfb) {
   constexpr int NE = DIM * (DIM + 1);
   cpab_ctx* ctx = b->ctx;
+  // both tables of the largest tessellation in LDS (measured on MI355X: the
+  // Trels alone in LDS with the As read from global memory, which doubles the
+  // occupancy of a 16 x 16 x 4-triangle batch, ran its best schedule 4 %
+  // slower, substeps being a quarter of its trips)
   const size_t lds = (size_t)2 * b->maxNC * NE * sizeof(double);
   auto k = k_integrate_batch<DIM, TESS, POW2>;
   allow_lds(k, lds);
