@@ -264,6 +264,9 @@ __device__ __forceinline__ bool trip(double* p, int& c, int& k, const double*& r
     for (int d = 0; d < DIM; d++) y[d] = p[d] + y[d] * h;
   }
   const int cy = cell_of<DIM, TESS, POW2, FAST>(y, g);
+  // (Measured on gfx950: the state update below written as selects instead
+  // of branches ran 3-4 % slower here and 11 % slower for 256 registration
+  // chains.)
   if (!fast) {
 #pragma unroll
     for (int d = 0; d < DIM; d++) p[d] = y[d];
@@ -299,8 +302,10 @@ __device__ __forceinline__ bool trip(double* p, int& c, int& k, const double*& r
 // (cell2_flat, no strict-interior test).  Measured on the one-chain
 // registration: ~770 instead of ~940 cycles a trip, +5 % at 40 slots, +11 %
 // at 24.  (Holding the table row in registers took the step kernel past 64
-// VGPRs.)  Same schedule and IEEE operations as `trip`.  Runs until t >= tb
-// and leaves row consistent with (c, k).
+// VGPRs; reading the row with scalar loads from a global copy of the tables,
+// the cell being wave-uniform here, measured 1-2 % slower than LDS.)  Same
+// schedule and IEEE operations as `trip`.  Runs until t >= tb and leaves row
+// consistent with (c, k).
 template <int DIM, int TESS, bool POW2>
 __device__ __forceinline__ int lone_cell(const double* y, const Geom& g) {
   if (DIM == 2 && TESS == TESS_I) return cell2_flat<POW2>(y, g);
