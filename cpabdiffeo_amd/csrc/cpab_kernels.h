@@ -411,13 +411,22 @@ __device__ __forceinline__ void integrate_range(int64_t begin, int64_t end, cons
       lone_trips<DIM, TESS, POW2>(p, c, k, t, nT, row, nfb, Ttab, Atab, h, nS, g);
       continue;
     }
-#pragma unroll 2
-    do {  // trips until some lane of the region finishes its point
+    // Trips until some lane of the region finishes its point, two per loop
+    // iteration with the exit test between them (written out: the compiler
+    // does not unroll a loop around a ballot), so the point's registers
+    // alternate instead of being copied back at the end of every trip.
+    for (;;) {
       if (trip<DIM, TESS, POW2, FAST>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) {
         if (MODE == 1) store_pt<DIM>(out, (int64_t)t * nPts + my, p);
         t++;
       }
-    } while (ballot(t >= nT) == 0);
+      if (ballot(t >= nT) != 0) break;
+      if (trip<DIM, TESS, POW2, FAST>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) {
+        if (MODE == 1) store_pt<DIM>(out, (int64_t)t * nPts + my, p);
+        t++;
+      }
+      if (ballot(t >= nT) != 0) break;
+    }
   }
 }
 

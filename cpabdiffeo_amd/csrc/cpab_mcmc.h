@@ -619,9 +619,12 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
           lone_trips<2, TESS, POW2>(p, c, k, t, tb, row, nfb, Ttab, Atab, h, nS, g);
           continue;
         }
-        do {
+        for (;;) {  // two trips per iteration, as in integrate_range
           if (trip<2, TESS, POW2, FAST>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) t++;
-        } while (ballot(t >= tb) == 0);
+          if (ballot(t >= tb) != 0) break;
+          if (trip<2, TESS, POW2, FAST>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) t++;
+          if (ballot(t >= tb) != 0) break;
+        }
       }
     }
   }
