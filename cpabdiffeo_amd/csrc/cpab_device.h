@@ -86,10 +86,11 @@ __device__ __forceinline__ int axis_box(double p, int nC, double inc, double rin
 template <bool POW2>
 __device__ __forceinline__ int axis_simplex(double p0, int nC, double inc, double rinc, double& frac) {
   if (POW2) {
-    double s = p0 * rinc;
-    double q = __builtin_trunc(s);
-    frac = s - q;
-    int i = (int)q;
+    // p0 >= 0 (clamped), so s >= 0: s - trunc(s) is exact and equals
+    // fract(s) = s - floor(s) (one v_fract_f64), and (int)s truncates.
+    const double s = p0 * rinc;
+    frac = __builtin_amdgcn_fract(s);
+    const int i = (int)s;
     return i < nC - 1 ? i : nC - 1;
   } else {
     double m = fmod_exact(p0, inc);
@@ -136,10 +137,9 @@ __device__ __forceinline__ int cell2(const double* p, const Geom& g) {
                         __builtin_amdgcn_fcmp(py, 0.0, 2) & __builtin_amdgcn_fcmp(py, g.top_y, 4);
     const uint64_t ex = __builtin_amdgcn_read_exec();
     if ((ok & ex) == ex) {
-      const double sx = px * g.rinc_x, sy = py * g.rinc_y;
-      const double qx = __builtin_trunc(sx), qy = __builtin_trunc(sy);
-      const double x = sx - qx, y = sy - qy;
-      const int c = ((int)qx + mul24((int)qy, g.nC0)) * 4;
+      const double sx = px * g.rinc_x, sy = py * g.rinc_y;  // > 0: fract as in axis_simplex
+      const double x = __builtin_amdgcn_fract(sx), y = __builtin_amdgcn_fract(sy);
+      const int c = ((int)sx + mul24((int)sy, g.nC0)) * 4;
       const double omx = 1.0 - x;
       if (x < y) return c + ((omx < y) ? 2 : 3);
       return c + ((omx < y) ? 1 : 0);
