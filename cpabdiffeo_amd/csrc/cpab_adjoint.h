@@ -121,6 +121,21 @@ __device__ __forceinline__ int rk2_substep(double* p, const double* At, double h
   return c;
 }
 
+// The same substep from p in its known cell c; returns the cell of the new
+// point (the cell the next substep or step starts in).
+template <int TESS, bool POW2>
+__device__ __forceinline__ int rk2_substep_in(double* p, int c, const double* At, double h, const Geom& g) {
+  const double* A = At + c * 6;
+  double v[2], m[2];
+  affine<2>(v, A, p);
+  m[0] = p[0] + h * v[0] / 2.;
+  m[1] = p[1] + h * v[1] / 2.;
+  affine<2>(v, A, m);
+  p[0] = p[0] + v[0] * h;
+  p[1] = p[1] + v[1] * h;
+  return cell2<TESS, POW2>(p, g);
+}
+
 // Active lanes with `need` add their accumulator G into row `cell` of the
 // wave's table `tab`, one lane at a time in lane order.  Which lanes are
 // active where the call sits is fixed by the data and the code, so the order
@@ -201,12 +216,14 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
     uint64_t* rb = FWD ? D.redo_bits + (int64_t)inst * nWords * nPts : nullptr;
     uint32_t sg = 0;
     uint64_t bits = 0;
+    // the cell of p, carried: an accepted step ends in the cell it started
+    // in, and each substep looks up the cell of its end point
+    int c = cell2<TESS, POW2>(p, g);
     for (int k = 0; k < nT; k++) {
       ck[((int64_t)k * nPts + idx) * 2] = p[0];
       ck[((int64_t)k * nPts + idx) * 2 + 1] = p[1];
       bool redo = true;
       if (FWD) {
-        const int c = cell2<TESS, POW2>(p, g);
         double pn[2];
         affine<2>(pn, Tr + c * 6, p);
         const int cn = cell2<TESS, POW2>(pn, g);
@@ -224,7 +241,10 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
       }
       if (redo) {
         nredo++;
-        for (int j = 0; j < nS; j++) sg = sig_mix(sg, rk2_substep<TESS, POW2>(p, At, h, g));
+        for (int j = 0; j < nS; j++) {
+          sg = sig_mix(sg, c);
+          c = rk2_substep_in<TESS, POW2>(p, c, At, h, g);
+        }
       }
     }
     if (D.sig) D.sig[ib + pix] = sg;
@@ -247,6 +267,10 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
     }
     // backward sweep
     int ca = -1, ct = -1;  // cells of the RK2 and Trel accumulators (-1: empty)
+    // the cell of the end point of the step being processed, carried back:
+    // an accepted step starts in the cell it ends in, and a redone step's
+    // start cell is the one its first substep looks up
+    int cq = c;
     double Ga[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double Gt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     // the checkpoint of the step before is loaded while a step is processed
@@ -258,11 +282,10 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
         qn[1] = ck[((int64_t)(k - 1) * nPts + idx) * 2 + 1];
       }
       bool redo = true;
-      int c0 = 0;
+      const int c0 = cq;  // an accepted step's cell
       if (FWD) {
         if (k == nT - 1 || (k & 63) == 63) bits = rb[(int64_t)(k >> 6) * nPts + idx];
         redo = (bits >> (k & 63)) & 1;
-        if (!redo) c0 = cell2<TESS, POW2>(q, g);
       }
       const bool any_redo = FWD ? (__ballot(redo) != 0) : true;  // wave-uniform
       double ps[NSM][2];
@@ -283,6 +306,7 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
           if (redo) {  // substep j of a redone step
             const double* pj = ps[j];
             const int c = cell2<TESS, POW2>(pj, g);
+            if (j == 0) cq = c;
             adj_flush(ca >= 0 && c != ca, ca, Ga, GA);
             if (c != ca) {
 #pragma unroll
