@@ -289,6 +289,14 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
       }
       const bool any_redo = FWD ? (__ballot(redo) != 0) : true;  // wave-uniform
       double ps[NSM][2];
+      // the substeps' start cells as the recompute looks them up, two 16-bit
+      // cells a register (the host keeps nC below 2^16), for the adjoint
+      // substeps below; with 16 substep registers the lookup is repeated
+      // there instead (the kernel would spill more than it saves)
+      constexpr bool kKeepCells = NSM <= 10;
+      uint32_t cs[(NSM + 1) / 2];
+#pragma unroll
+      for (int i = 0; i < (NSM + 1) / 2; i++) cs[i] = 0u;
       if (any_redo) {
         double r[2] = {q[0], q[1]};
 #pragma unroll
@@ -296,7 +304,10 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
           if (j < nS) {
             ps[j][0] = r[0];
             ps[j][1] = r[1];
-            if (redo) rk2_substep<TESS, POW2>(r, At, h, g);
+            if (redo) {
+              const int cj = rk2_substep<TESS, POW2>(r, At, h, g);
+              if (kKeepCells) cs[j >> 1] |= (uint32_t)cj << ((j & 1) * 16);
+            }
           }
         }
       }
@@ -305,7 +316,7 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
         if (j < nS && (any_redo || j == 0)) {  // wave-uniform
           if (redo) {  // substep j of a redone step
             const double* pj = ps[j];
-            const int c = cell2<TESS, POW2>(pj, g);
+            const int c = kKeepCells ? (int)((cs[j >> 1] >> ((j & 1) * 16)) & 0xffffu) : cell2<TESS, POW2>(pj, g);
             if (j == 0) cq = c;
             adj_flush(ca >= 0 && c != ca, ca, Ga, GA);
             if (c != ca) {

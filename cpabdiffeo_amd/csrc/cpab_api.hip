@@ -634,6 +634,8 @@ int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts
   int r = make_geom(ctx, 2, tess, nC0, nC1, 0, ix, iy, 1.0, &g, &pow2);
   if (r) return r;
   if ((r = check_nC(ctx, 2, tess, nC, nC0, nC1, 0))) return r;
+  // (the tables' LDS bound below keeps nC far lower; the kernel packs cells in 16 bits)
+  if (nC > 65535) return fail(ctx, CPAB_ERR_UNSUPPORTED, "the adjoint takes at most 65535 cells");
   if (nInst < 1 || nInst > 65535) return fail(ctx, CPAB_ERR_ARG, "nInst must be in [1, 65535]");
   if (nPts < 0 || nT < 0 || nS < 0) return fail(ctx, CPAB_ERR_ARG, "bad sizes");
   if (nS > ADJ_NSMAX) return fail(ctx, CPAB_ERR_UNSUPPORTED, "nStepsODE > %d is not supported by the adjoint", ADJ_NSMAX);
