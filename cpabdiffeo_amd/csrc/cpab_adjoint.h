@@ -28,12 +28,12 @@
 // Schedule, one lane per point: the forward sweep writes the point at the
 // start of every time step to a checkpoint buffer (nT points per point) and,
 // for calc_T, one bit per step recording whether the step was redone; the
-// backward sweep walks the time steps in reverse (the next checkpoint loaded
-// while a step is processed), recomputes a redone step's nS substep start
-// points from its checkpoint into registers (NSM of them, unrolled so they
-// stay in registers), then runs the step's adjoint units in reverse: a
-// redone step's substeps j = nS-1 .. 0, an accepted step's Trel step at
-// j = 0 (a wave whose lanes all accepted runs only that unit).
+// backward sweep walks the time steps in reverse (each lane its own, for
+// calc_T's map: accepted steps advance together, lanes at a redone step wait
+// for others and run theirs together), recomputes a redone step's nS
+// substep start points from its checkpoint into registers (NSM of them,
+// unrolled so they stay in registers), then runs its substeps' adjoints
+// j = nS-1 .. 0; an accepted step is one Trel adjoint unit.
 //
 // Accumulation, bit-reproducible: each lane keeps two register
 // accumulators, dJ/dA of its current RK2 cell and dJ/dTrel of its current
@@ -273,93 +273,122 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
     int cq = c;
     double Ga[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double Gt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    // the checkpoint of the step before is loaded while a step is processed
-    double qn[2] = {ck[((int64_t)(nT - 1) * nPts + idx) * 2], ck[((int64_t)(nT - 1) * nPts + idx) * 2 + 1]};
-    for (int k = nT - 1; k >= 0; k--) {
-      const double q[2] = {qn[0], qn[1]};
-      if (k > 0) {
-        qn[0] = ck[((int64_t)(k - 1) * nPts + idx) * 2];
-        qn[1] = ck[((int64_t)(k - 1) * nPts + idx) * 2 + 1];
-      }
-      bool redo = true;
-      const int c0 = cq;  // an accepted step's cell
-      if (FWD) {
-        if (k == nT - 1 || (k & 63) == 63) bits = rb[(int64_t)(k >> 6) * nPts + idx];
-        redo = (bits >> (k & 63)) & 1;
-      }
-      const bool any_redo = FWD ? (__ballot(redo) != 0) : true;  // wave-uniform
+    // the substeps' start cells as the recompute looks them up, two 16-bit
+    // cells a register (the host keeps nC below 2^16), for the adjoint
+    // substeps; with 16 substep registers the lookup is repeated there
+    // instead (the kernel would spill more than it saves)
+    constexpr bool kKeepCells = NSM <= 10;
+    // One redone step from its start point q, in reverse: the recompute of
+    // its substep start points (and cells), then the adjoint of each
+    // substep.  Sets cq to the cell of q.
+    auto redone_step = [&](const double* q) {
       double ps[NSM][2];
-      // the substeps' start cells as the recompute looks them up, two 16-bit
-      // cells a register (the host keeps nC below 2^16), for the adjoint
-      // substeps below; with 16 substep registers the lookup is repeated
-      // there instead (the kernel would spill more than it saves)
-      constexpr bool kKeepCells = NSM <= 10;
       uint32_t cs[(NSM + 1) / 2];
 #pragma unroll
       for (int i = 0; i < (NSM + 1) / 2; i++) cs[i] = 0u;
-      if (any_redo) {
-        double r[2] = {q[0], q[1]};
+      double r[2] = {q[0], q[1]};
 #pragma unroll
-        for (int j = 0; j < NSM; j++) {
-          if (j < nS) {
-            ps[j][0] = r[0];
-            ps[j][1] = r[1];
-            if (redo) {
-              const int cj = rk2_substep<TESS, POW2>(r, At, h, g);
-              if (kKeepCells) cs[j >> 1] |= (uint32_t)cj << ((j & 1) * 16);
-            }
-          }
+      for (int j = 0; j < NSM; j++) {
+        if (j < nS) {
+          ps[j][0] = r[0];
+          ps[j][1] = r[1];
+          const int cj = rk2_substep<TESS, POW2>(r, At, h, g);
+          if (kKeepCells) cs[j >> 1] |= (uint32_t)cj << ((j & 1) * 16);
         }
       }
 #pragma unroll
       for (int j = NSM - 1; j >= 0; j--) {
-        if (j < nS && (any_redo || j == 0)) {  // wave-uniform
-          if (redo) {  // substep j of a redone step
-            const double* pj = ps[j];
-            const int c = kKeepCells ? (int)((cs[j >> 1] >> ((j & 1) * 16)) & 0xffffu) : cell2<TESS, POW2>(pj, g);
-            if (j == 0) cq = c;
-            adj_flush(ca >= 0 && c != ca, ca, Ga, GA);
-            if (c != ca) {
+        if (j < nS) {
+          const double* pj = ps[j];
+          const int c = kKeepCells ? (int)((cs[j >> 1] >> ((j & 1) * 16)) & 0xffffu) : cell2<TESS, POW2>(pj, g);
+          if (j == 0) cq = c;
+          adj_flush(ca >= 0 && c != ca, ca, Ga, GA);
+          if (c != ca) {
 #pragma unroll
-              for (int e = 0; e < 6; e++) Ga[e] = 0.0;
-              ca = c;
-            }
-            const double* A = At + c * 6;
-            double v1[2], m[2];
-            affine<2>(v1, A, pj);
-            m[0] = pj[0] + h * v1[0] / 2.;
-            m[1] = pj[1] + h * v1[1] / 2.;
-            const double g2x = h * lam[0], g2y = h * lam[1];
-            const double gmx = A[0] * g2x + A[3] * g2y, gmy = A[1] * g2x + A[4] * g2y;
-            const double g1x = 0.5 * h * gmx, g1y = 0.5 * h * gmy;
-            lam[0] = lam[0] + gmx + (A[0] * g1x + A[3] * g1y);
-            lam[1] = lam[1] + gmy + (A[1] * g1x + A[4] * g1y);
-            Ga[0] += g2x * m[0] + g1x * pj[0];
-            Ga[1] += g2x * m[1] + g1x * pj[1];
-            Ga[2] += g2x + g1x;
-            Ga[3] += g2y * m[0] + g1y * pj[0];
-            Ga[4] += g2y * m[1] + g1y * pj[1];
-            Ga[5] += g2y + g1y;
-          } else if (FWD && j == 0) {  // the accepted Trel step
-            adj_flush(ct >= 0 && c0 != ct, ct, Gt, GT);
-            if (c0 != ct) {
-#pragma unroll
-              for (int e = 0; e < 6; e++) Gt[e] = 0.0;
-              ct = c0;
-            }
-            const double* T = Tr + c0 * 6;
-            Gt[0] += lam[0] * q[0];
-            Gt[1] += lam[0] * q[1];
-            Gt[2] += lam[0];
-            Gt[3] += lam[1] * q[0];
-            Gt[4] += lam[1] * q[1];
-            Gt[5] += lam[1];
-            const double l0 = T[0] * lam[0] + T[3] * lam[1];
-            const double l1 = T[1] * lam[0] + T[4] * lam[1];
-            lam[0] = l0;
-            lam[1] = l1;
+            for (int e = 0; e < 6; e++) Ga[e] = 0.0;
+            ca = c;
           }
+          const double* A = At + c * 6;
+          double v1[2], m[2];
+          affine<2>(v1, A, pj);
+          m[0] = pj[0] + h * v1[0] / 2.;
+          m[1] = pj[1] + h * v1[1] / 2.;
+          const double g2x = h * lam[0], g2y = h * lam[1];
+          const double gmx = A[0] * g2x + A[3] * g2y, gmy = A[1] * g2x + A[4] * g2y;
+          const double g1x = 0.5 * h * gmx, g1y = 0.5 * h * gmy;
+          lam[0] = lam[0] + gmx + (A[0] * g1x + A[3] * g1y);
+          lam[1] = lam[1] + gmy + (A[1] * g1x + A[4] * g1y);
+          Ga[0] += g2x * m[0] + g1x * pj[0];
+          Ga[1] += g2x * m[1] + g1x * pj[1];
+          Ga[2] += g2x + g1x;
+          Ga[3] += g2y * m[0] + g1y * pj[0];
+          Ga[4] += g2y * m[1] + g1y * pj[1];
+          Ga[5] += g2y + g1y;
         }
+      }
+    };
+    if (FWD) {
+      // Each lane walks its own steps back.  A trip either takes one
+      // accepted step on every lane whose current step is accepted, or
+      // (a redo trip) runs the redone step of every lane parked at one;
+      // redo trips run once kRedoBatch lanes are parked or nothing else can
+      // move, so a redone step costs its wave one pass of its nS substeps
+      // shared by several lanes rather than one pass per lane and step.
+      // Every choice is a ballot of the data: the sums keep a fixed order.
+      constexpr int kRedoBatch = 8;
+      int k = nT - 1;  // this lane's step (-1: done)
+      double q[2] = {ck[((int64_t)k * nPts + idx) * 2], ck[((int64_t)k * nPts + idx) * 2 + 1]};
+      bits = rb[(int64_t)(k >> 6) * nPts + idx];
+      bool redo = (bits >> (k & 63)) & 1;
+      auto next_step = [&]() {  // the step before: its start point (checkpoint) and decision bit
+        if (--k >= 0) {
+          q[0] = ck[((int64_t)k * nPts + idx) * 2];
+          q[1] = ck[((int64_t)k * nPts + idx) * 2 + 1];
+          if ((k & 63) == 63) bits = rb[(int64_t)(k >> 6) * nPts + idx];
+          redo = (bits >> (k & 63)) & 1;
+        }
+      };
+      for (;;) {
+        const uint64_t parked = __ballot(k >= 0 && redo);
+        const uint64_t ready = __ballot(k >= 0 && !redo);
+        if ((parked | ready) == 0) break;
+        if (ready == 0 || __popcll(parked) >= kRedoBatch) {
+          if (k >= 0 && redo) {
+            redone_step(q);
+            next_step();
+          }
+        } else if (k >= 0 && !redo) {  // the accepted Trel step: it starts in the cell it ends in
+          const int c0 = cq;
+          adj_flush(ct >= 0 && c0 != ct, ct, Gt, GT);
+          if (c0 != ct) {
+#pragma unroll
+            for (int e = 0; e < 6; e++) Gt[e] = 0.0;
+            ct = c0;
+          }
+          const double* T = Tr + c0 * 6;
+          Gt[0] += lam[0] * q[0];
+          Gt[1] += lam[0] * q[1];
+          Gt[2] += lam[0];
+          Gt[3] += lam[1] * q[0];
+          Gt[4] += lam[1] * q[1];
+          Gt[5] += lam[1];
+          const double l0 = T[0] * lam[0] + T[3] * lam[1];
+          const double l1 = T[1] * lam[0] + T[4] * lam[1];
+          lam[0] = l0;
+          lam[1] = l1;
+          next_step();
+        }
+      }
+    } else {  // calc_T_simple's map: every step is nS substeps, the same on every lane
+      // (the checkpoint of the step before is loaded while a step is processed)
+      double qn[2] = {ck[((int64_t)(nT - 1) * nPts + idx) * 2], ck[((int64_t)(nT - 1) * nPts + idx) * 2 + 1]};
+      for (int k = nT - 1; k >= 0; k--) {
+        const double q[2] = {qn[0], qn[1]};
+        if (k > 0) {
+          qn[0] = ck[((int64_t)(k - 1) * nPts + idx) * 2];
+          qn[1] = ck[((int64_t)(k - 1) * nPts + idx) * 2 + 1];
+        }
+        redone_step(q);
       }
     }
     adj_flush(ca >= 0, ca, Ga, GA);
