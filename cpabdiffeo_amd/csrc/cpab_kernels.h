@@ -271,7 +271,7 @@ __device__ __forceinline__ bool trip(double* p, int& c, int& k, const double*& r
 #pragma unroll
     for (int d = 0; d < DIM; d++) p[d] = y[d];
     const bool done = --k == 0;
-    if (done | (cy != c)) row = (done ? Ttab : Atab) + mul24(cy, NE);
+    row = (done ? Ttab : Atab) + mul24(cy, NE);  // (the same row when the cell did not change)
     c = cy;
     return done;
   }
