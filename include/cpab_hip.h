@@ -110,7 +110,9 @@ int cpab_calc_grad_theta(cpab_ctx *ctx, int dim, int tess, int nC, const double 
  * sum_x lam(x)^T dT(x)/dA, A the integrated (signed) tables As (nInst, nC, 6).
  * dtheta follows as B^T gradA (times the sign).  One checkpoint per time step
  * in a library workspace (16 * nTimeSteps bytes per point); the substeps are
- * recomputed.  nStepsODE <= 16.  pos_out (optional): the end points. */
+ * recomputed.  nStepsODE <= 16.  The tables and per-wave gradient tables
+ * live in LDS: nC up to 1706 here, 853 for the calc_T map below
+ * (CPAB_ERR_UNSUPPORTED beyond).  pos_out (optional): the end points. */
 int cpab_calc_T_simple_vjp(cpab_ctx *ctx, int tess, int nC, int nInst, const double *pos0,
                            const double *lam, const double *As, double dt, int nTimeSteps,
                            int nStepsODE, int64_t nPts, int nC0, int nC1, double inc_x,
