@@ -395,8 +395,8 @@ def main():
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: this build runs on a single MI355X "
-              "(chains and point sets shard across GPUs without communication; see DESIGN.md)",
+        print("bench.py measures one GPU only: the library runs on a single MI355X "
+              "(chains and point sets would shard across GPUs without communication)",
               file=sys.stderr)
         sys.exit(2)
     if args.section:
