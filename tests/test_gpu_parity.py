@@ -324,3 +324,34 @@ def test_grad_theta_bitexact(gpu, orc, name, lo, hi, nCs, tess):
     assert np.array_equal(host(out), wpos)
     g = host(grad)
     assert np.array_equal(g, wgrad), np.abs(g - wgrad).max()
+
+
+@pytest.mark.parametrize("nT,nS", [(1, 10), (7, 1), (7, 0), (13, 3), (100, 1)])
+def test_calc_T_step_counts_bitexact(gpu, orc, nT, nS):
+    """Odd and single step counts (the trip loop runs two trips between exit
+    tests) and zero, one or few substeps (zero: a rejected proposal leaves the
+    point where it was, calc_2Dto2D.cu:395-401 with an empty solveODE loop),
+    in two launch forms (20 000 points: one per lane, the general lookup;
+    600 000: two per lane with refill and the wave-uniform fast lookup): end
+    points, fallback counts and the trajectory bit-identical."""
+    torch = gpu
+    say("calc_T nT", nT, "nS", nS)
+    for n in (20000, 600000):
+        sp = space_for([0, 0], [1, 1], (4, 4), "I")
+        g = sp.tessellation.geom()
+        Av = random_field(sp, 40 + nS)
+        pts = np.random.default_rng(41 + nT).uniform(0, 1, (n, 2))
+        sA, T = orc.prep(2, Av.reshape(sp.nC, -1), -1, 0.05)
+        want, nfb_w, _ = orc.calc_T(g, pts, T, sA, 0.05, nT, nS, probe=True)
+        out = torch.zeros(pts.shape, dtype=torch.float64, device="cuda")
+        nfb = torch.zeros(n, dtype=torch.int32, device="cuda")
+        tdev, adev = dev(torch, T), dev(torch, sA)
+        sp._gpu_calcs.calc_T(None, None, tdev, adev, dev(torch, pts), 0.05, nT, nS, out, 2, 2, sp.nCs, sp.incs,
+                             n_fallback=nfb)
+        assert np.array_equal(host(out), want), (n, np.abs(host(out) - want).max())
+        assert np.array_equal(host(nfb), nfb_w)
+        assert nfb_w.sum() > 0
+        if n == 20000:
+            traj = sp._gpu_calcs.calc_trajectory(None, None, tdev, adev, dev(torch, pts), 0.05, nT, nS, 2, 2,
+                                                 sp.nCs, sp.incs)
+            assert np.array_equal(host(traj), orc.calc_trajectory(g, pts, T, sA, 0.05, nT, nS))
