@@ -355,3 +355,30 @@ def test_calc_T_step_counts_bitexact(gpu, orc, nT, nS):
             traj = sp._gpu_calcs.calc_trajectory(None, None, tdev, adev, dev(torch, pts), 0.05, nT, nS, 2, 2,
                                                  sp.nCs, sp.incs)
             assert np.array_equal(host(traj), orc.calc_trajectory(g, pts, T, sA, 0.05, nT, nS))
+
+
+@pytest.mark.parametrize("n", [20000, 600000])
+def test_calc_T_outside_wall_and_special_points(gpu, orc, n):
+    """Start points outside the domain, exactly on cell walls and diagonals,
+    and NaN / +-inf / +-1e300 / +-0 / subnormal coordinates, through calc_T in
+    both launch forms (600 000 points take the wave-uniform fast lookup, whose
+    waves with a point outside the domain fall back to the general lookup):
+    end points (NaN where the oracle has NaN) and fallback counts identical."""
+    torch = gpu
+    say("calc_T outside / wall / special points", n)
+    sp = space_for([0, 0], [1, 1], (4, 4), "I")
+    g = sp.tessellation.geom()
+    Av = random_field(sp, 50)
+    rng = np.random.default_rng(51)
+    pts = lookup_points(sp.tessellation, rng, n)
+    pts = pts[rng.permutation(len(pts))][:n]
+    for sign in (1, -1):
+        sA, T = orc.prep(2, Av.reshape(sp.nC, -1), sign, 0.01)
+        want, nfb_w, _ = orc.calc_T(g, pts, T, sA, 0.01, 100, 10, probe=True)
+        out = torch.zeros(pts.shape, dtype=torch.float64, device="cuda")
+        nfb = torch.zeros(len(pts), dtype=torch.int32, device="cuda")
+        sp._gpu_calcs.calc_T(None, None, dev(torch, T), dev(torch, sA), dev(torch, pts), 0.01, 100, 10, out, 2, 2,
+                             sp.nCs, sp.incs, n_fallback=nfb)
+        got = host(out)
+        assert np.array_equal(got, want, equal_nan=True), (sign, np.nanmax(np.abs(got - want)))
+        assert np.array_equal(host(nfb), nfb_w)
