@@ -33,7 +33,13 @@ def small_batch(gpu):
     plan._nc = [spaces[g][0].nC for g, _, _ in insts]
     As = torch.from_numpy(np.concatenate([Av.reshape(-1, 6) for _, Av, _ in insts])).cuda()
     rng = np.random.default_rng(0)
-    pts = torch.from_numpy(rng.uniform(0, 1, (plan.nPts, 2))).cuda()
+    p = rng.uniform(0, 1, (plan.nPts, 2))
+    # a few start points outside the domain and on cell walls / diagonals
+    # (multiples of a quarter of every tessellation's increment)
+    k = rng.permutation(plan.nPts)[:plan.nPts // 10]
+    p[k[:len(k) // 2]] = rng.uniform(-0.2, 1.2, (len(k) // 2, 2))
+    p[k[len(k) // 2:]] = rng.integers(0, 65, (len(k) - len(k) // 2, 2)) / 64.0
+    pts = torch.from_numpy(p).cuda()
     return dict(spaces=spaces, insts=insts, plan=plan, As=As, pts=pts)
 
 
