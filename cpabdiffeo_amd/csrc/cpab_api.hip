@@ -1509,6 +1509,7 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   D.lp_prop = D.lp + q.nChains;
   D.wlp = q.wlp;
   D.J = q.prior_prec;
+  D.integrate_corner = q.integrate_corner != 0;
   D.done_slot = reinterpret_cast<unsigned int*>(r->scratch + o_ds);
   D.stop = reinterpret_cast<int64_t*>(r->scratch + o_ds + n_ds);
   D.done = reinterpret_cast<unsigned int*>(D.stop + q.nChains);

@@ -135,6 +135,7 @@ struct McmcDesc {
   double* lp;           // (nChains,) log-prior of the current theta (with J)
   double* lp_prop;      // (nChains, K) log-prior of each armed proposal (with J)
   Geom g;
+  int integrate_corner;  // 1: the domain corner is integrated even when its sample is fixed
 };
 
 // LDS of the last block's epilogue (doubles): ll', fallback count,
@@ -482,6 +483,93 @@ __device__ inline bool mcmc_accepts(const McmcDesc& D, int ch, int64_t step, int
   return D.inverted ? (lu > lr) : (lu < lr);
 }
 
+// x^n for x >= 1, rounded up (each product inflated past its rounding error).
+__device__ inline double pow_up(double x, int n) {
+  constexpr double up = 1.0 + 0x1p-40;
+  double r = 1.0;
+  for (; n > 0; n >>= 1) {
+    if (n & 1) r = r * x * up;
+    x = x * x * up;
+  }
+  return r;
+}
+
+// Whether the pixel at the domain corner (0, 0) can be left out of the
+// integration because its sample is src(0, 0) bit for bit, whatever path
+// the trips take.  With zero velocity on the boundary the corner's velocity
+// is zero up to the rounding of B theta, so its point stays within ~1e-12 px
+// of (0, 0) -- yet it falls back to the RK2 substeps at nearly every step
+// (~1 100 dependent trips where other pixels need ~104), which would bound
+// the launch.
+//  * The path: every point it visits lies within rho of (0, 0), by
+//    induction over the trips.  A point within rho (below half the
+//    increments) looks up a cell of box (0, 0) that touches the corner: the
+//    box itself (type II); of type I's triangles 0 (top) or 3 (left) -- inside
+//    the box 1 - x > y rules out 1 and 2, outside the left rule gives 0 or 3
+//    and the top rule 0 (cell2, calc_2Dto2D.cu:38-153).  With
+//    gT, tT, a, b the largest |T_r0| + |T_r1|, |T_r2|, |A_r0| + |A_r1|,
+//    |A_r2| over those cells' rows: a Trel step maps rho to gT rho + tT; an
+//    RK2 substep (v = A[p;1], m = p + h/2 v, p' = p + h A[m;1]) to
+//    rho (1 + h a (1 + h/2 a)) + h b (1 + h/2 a); each factor inflated by
+//    2^-40 for the rounding of a trip's ten IEEE operations, 2^-1000 added
+//    per step for underflow.  Over nT steps: rho_T <= tau nT g^nT, g >= 1
+//    the larger growth of a step (one Trel step, or nS substeps), tau its
+//    larger offset.
+//  * The sample (sample2, FP32 weights): for |x|, |y| <= rho <= 2^-27 the
+//    weights are exactly 0 or 1, or below 2^-25 (x in [0, rho): fx = x,
+//    1 - fx rounds to 1; x in [-rho, 0): x - floor(x) rounds to 1, and the
+//    pixel at -1 is outside, zero), so each of the sample's two additions is
+//    v00 (or 0) plus a term of at most rho M (1 + 2^-20), M the largest
+//    |v| of the four pixels; below a quarter of v00's FP32 ulp that term
+//    rounds away.  So w = v00 when v00 is a normal number and
+//    4 rho M (1 + 2^-20) <= ulp(v00).
+// The path depends on the slot's tables, the sample on src: every lane of
+// the wave computes the same answer.  NaN or infinite tables give false.
+template <int TESS>
+__device__ inline bool corner_sample_is_src00(const double* Ttab, const double* Atab, const McmcDesc& D) {
+  constexpr double up = 1.0 + 0x1p-40;
+  const int ncell = TESS == TESS_I ? 2 : 1;
+  double gT = 0.0, tT = 0.0, a = 0.0, b = 0.0, probe = 0.0;
+  for (int i = 0; i < ncell; i++) {
+    const int c = 3 * i;  // cells 0 and 3 (type I), 0 (type II)
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      const double* T = Ttab + c * 6 + r * 3;
+      const double* A = Atab + c * 6 + r * 3;
+      const double g1 = fabs(T[0]) + fabs(T[1]), t1 = fabs(T[2]);
+      const double a1 = fabs(A[0]) + fabs(A[1]), b1 = fabs(A[2]);
+      gT = fmax(gT, g1);
+      tT = fmax(tT, t1);
+      a = fmax(a, a1);
+      b = fmax(b, b1);
+      probe += g1 + t1 + a1 + b1;  // (fmax drops a NaN; the sum keeps it)
+    }
+  }
+  if (!(probe <= 1e300) || !(D.h > 0.0) || D.nT < 0 || D.nS < 0) return false;
+  gT *= up;
+  tT *= up;
+  a *= up;
+  b *= up;
+  const double h = D.h, ga = (1.0 + 0.5 * h * a) * up;
+  const double gs = (1.0 + h * a * ga) * up * up, ts = h * b * ga * up * up;  // one substep
+  const double gsn = pow_up(gs, D.nS);                                         // nS substeps
+  const double tsn = ts * (double)D.nS * gsn * up;
+  const double g = fmax(fmax(gT * up, gsn), 1.0);
+  const double tau = fmax(tT * up, tsn) + 0x1p-1000;
+  const double rho = tau * (double)D.nT * pow_up(g, D.nT) * up;
+  if (!(rho <= 0x1p-27) || !(rho < 0.5 * fmin(D.g.inc_x, D.g.inc_y))) return false;
+  const int W = D.W, H = D.H;
+  const float v00 = D.src[0];
+  const float v01 = W > 1 ? D.src[1] : 0.f;
+  const float v10 = H > 1 ? D.src[W] : 0.f;
+  const float v11 = (W > 1 && H > 1) ? D.src[W + 1] : 0.f;
+  const double m = fmax(fmax(fabs((double)v00), fabs((double)v01)), fmax(fabs((double)v10), fabs((double)v11)));
+  if (!(fabs((double)v00) >= 0x1p-126) || !(m <= 3.5e38)) return false;
+  int e;
+  (void)frexp((double)v00, &e);  // |v00| in [2^(e-1), 2^e): its FP32 ulp is 2^(e-24)
+  return 4.0 * rho * m * (1.0 + 0x1p-20) <= ldexp(1.0, e - 24);
+}
+
 // One launch evaluates up to K proposals of every chain: grid (K * nBlocks,
 // nChains), or (nChains, K * nBlocks) for several chains.  Dispatch order:
 // the first pixel block of every slot (it holds the domain corner, whose
@@ -556,6 +644,11 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
     const int nT = D.nT, nS = D.nS;
     const double h = D.h;
     if (begin < nPts) {
+      // The domain corner's sample, fixed by a bound on its path: its lane
+      // takes no trips (one pixel per lane only, so every lane's sum keeps
+      // its terms and order; its rejected Trel proposals are not counted).
+      const bool corner_fixed =
+          begin == 0 && D.ptsPerWave == 64 && !D.integrate_corner && corner_sample_is_src00<TESS>(Ttab, Atab, D);
       // Abort checkpoints: with one pixel per lane every lane runs steps 0..nT
       // once, so the wave can stop at fixed steps when its slot lies above a
       // rejection found meanwhile (no cost per trip: the trip loop's bound
@@ -597,6 +690,12 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
               row = Ttab + mul24(c, 6);
               t = 0;
               k = 0;
+              if (corner_fixed && pix == 0) {  // its sample: src(0, 0) (corner_sample_is_src00)
+                const double e = (double)D.src[0] - (double)D.dst[0];
+                acc += e * e;
+                pxw += 1.0;
+                active = false;
+              }
             }
           }
           next += __popcll(idle);

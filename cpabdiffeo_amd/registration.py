@@ -39,7 +39,7 @@ class RegDesc(ctypes.Structure):
                 ("ll", ctypes.c_void_p), ("n_accepted", ctypes.c_void_p), ("step", ctypes.c_void_p),
                 ("trace", ctypes.c_void_p), ("trace_len", ctypes.c_int), ("n_fallback", ctypes.c_void_p),
                 ("spec_depth", ctypes.c_int), ("stats", ctypes.c_void_p), ("pts_per_wave", ctypes.c_int),
-                ("wlp", ctypes.c_double), ("prior_prec", ctypes.c_void_p)]
+                ("wlp", ctypes.c_double), ("prior_prec", ctypes.c_void_p), ("integrate_corner", ctypes.c_int)]
 
 
 def pixel_basis(space_unit, n):
@@ -94,11 +94,13 @@ class DeviceMetropolis:
     ratio wlp (lp' - lp), lp = -0.5 theta^T Sigma_CPA^-1 theta, to the
     acceptance test as Register does (Metropolis.py:64-70,
     Register.py:224-248, wlp = 1e-4 there); prior_prec overrides the
-    precision matrix."""
+    precision matrix.  integrate_corner: integrate the domain-corner pixel
+    even when a bound on its path fixes its sample to src(0, 0) (the chain is
+    the same either way; only the fallback counts and stats include it)."""
 
     def __init__(self, problem, nChains=1, scale=0.01, diagonal=False, use_ave_ll=True,
                  inverted_acceptance=False, seed=0, trace_len=0, count_fallbacks=False, spec_depth=1,
-                 pts_per_wave=0, use_prior=False, wlp=1e-4, prior_prec=None):
+                 pts_per_wave=0, use_prior=False, wlp=1e-4, prior_prec=None, integrate_corner=False):
         torch = require_gpu()
         self.problem = problem
         self.nChains = nChains
@@ -144,7 +146,7 @@ class DeviceMetropolis:
                        self.step_counter.data_ptr(), self.trace.data_ptr() if trace_len else None, int(trace_len),
                        self.n_fallback.data_ptr() if count_fallbacks else None, int(spec_depth),
                        self.stats.data_ptr(), int(pts_per_wave), self.wlp,
-                       self._J.data_ptr() if use_prior else None)
+                       self._J.data_ptr() if use_prior else None, int(bool(integrate_corner)))
         self._ctx = Context.get(0)
         self._ctx.bind_stream(torch.cuda.current_stream().cuda_stream)
         h = ctypes.c_void_p()

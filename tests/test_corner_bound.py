@@ -1,0 +1,99 @@
+"""The domain-corner shortcut of the registration step, checked on the CPU.
+
+The step kernel leaves the pixel at (0, 0) out of the integration when a
+bound on its path (from the tables of the cells of box (0, 0)) shows that
+its FP32 bilinear sample is src(0, 0) bit for bit.  The bound is restated
+here as the kernel forms it, and checked against the oracle: over prior
+draws at several scales, every step of the corner's trajectory lies within
+it, and where it allows the shortcut the oracle's remap at the trajectory's
+end equals src(0, 0) bit for bit.  (The GPU side -- the same chain with and
+without the shortcut -- is tests/test_gpu_mcmc.py.)
+"""
+import numpy as np
+import pytest
+
+from cpabdiffeo_amd import configs
+from cpabdiffeo_amd.registration import pixel_basis
+
+UP = 1.0 + 2.0 ** -40
+
+
+def pow_up(x, n):
+    r = 1.0
+    while n > 0:
+        if n & 1:
+            r = r * x * UP
+        x = x * x * UP
+        n >>= 1
+    return r
+
+
+def corner_bound(T, A, h, nT, nS, cells=(0, 3)):
+    """|x|, |y| of every point of the corner's path are at most this: the
+    path only looks up the cells touching the corner (type I triangles 0
+    and 3 of box (0, 0))."""
+    T = np.asarray(T, np.float64)[list(cells)].reshape(-1, 3)
+    A = np.asarray(A, np.float64)[list(cells)].reshape(-1, 3)
+    gT = np.max(np.abs(T[:, 0]) + np.abs(T[:, 1])) * UP
+    tT = np.max(np.abs(T[:, 2])) * UP
+    a = np.max(np.abs(A[:, 0]) + np.abs(A[:, 1])) * UP
+    b = np.max(np.abs(A[:, 2])) * UP
+    ga = (1.0 + 0.5 * h * a) * UP
+    gs = (1.0 + h * a * ga) * UP * UP
+    ts = h * b * ga * UP * UP
+    gsn = pow_up(gs, nS)
+    tsn = ts * nS * gsn * UP
+    g = max(gT * UP, gsn, 1.0)
+    tau = max(tT * UP, tsn) + 2.0 ** -1000
+    return tau * nT * pow_up(g, nT) * UP
+
+
+def sample_fixed(rho, img, inc):
+    """Whether |x|, |y| <= rho leaves the FP32 sample at src(0, 0)."""
+    if not (rho <= 2.0 ** -27 and rho < 0.5 * inc):
+        return False
+    v = [abs(float(img[0, 0])), abs(float(img[0, 1])), abs(float(img[1, 0])), abs(float(img[1, 1]))]
+    if not v[0] >= 2.0 ** -126:
+        return False
+    e = np.frexp(v[0])[1]
+    return 4.0 * rho * max(v) * (1.0 + 2.0 ** -20) <= 2.0 ** (e - 24)
+
+
+@pytest.mark.parametrize("n,nT", [(256, 100), (64, 50)])
+def test_corner_path_within_bound_and_sample_fixed(orc, n, nT):
+    nS, dt = 10, 1.0 / nT
+    sp_unit = configs.unit_space((4, 4), "I")
+    sp = configs.pixel_space(n, (4, 4), "I")
+    g = sp.tessellation.geom()
+    B = pixel_basis(sp_unit, n)
+    cov = sp_unit.se_prior_cov(0.1, 1.0)
+    img = configs.blob_image(n, 40, seed=0)
+    rng = np.random.default_rng(n)
+    fixed = 0
+    for scale in (0.1, 0.3, 1.0, 3.0):
+        for _ in range(6):
+            theta = scale * sp_unit.sample_theta(cov, rng)
+            Av = (B @ theta).reshape(sp.nC, 6)
+            sA, T = orc.prep(2, Av, -1, dt)
+            traj = orc.calc_trajectory(g, np.zeros((1, 2)), T, sA, dt, nT, nS)[:, 0, :]
+            rho = corner_bound(T, sA, dt / nS, nT, nS)
+            assert np.abs(traj).max() <= rho, (scale, np.abs(traj).max(), rho)
+            if sample_fixed(rho, img, sp.tessellation.incs[0]):
+                fixed += 1
+                pts = np.zeros((n * n, 2))
+                pts[0] = traj[-1]
+                assert orc.remap2d(pts, img)[0, 0] == img[0, 0]
+    assert fixed >= 12  # the bound is tight enough to fix the sample for moderate draws
+
+
+def test_corner_bound_grows_with_the_corner_velocity():
+    """A corner cell with a translation (velocity at the corner) of size
+    eps gives a bound above the eps * t the point then travels."""
+    T = np.tile(np.array([1.0, 0.0, 0.0, 0.0, 1.0, 0.0]), (4, 1))
+    A = np.zeros((4, 6))
+    for eps in (1e-15, 1e-10, 1e-6):
+        A[0, 2] = eps
+        T[0, 2] = eps * 0.01
+        rho = corner_bound(T, A, 0.001, 100, 10)
+        assert rho >= eps * 1.0
+        assert rho <= eps * 1.1 + 1e-300

@@ -125,7 +125,7 @@ def test_fallback_counter_matches_calc_T(problem, orc, gpu):
     from cpabdiffeo_amd.registration import DeviceMetropolis
     say("fallback counter")
     p = problem
-    mh = DeviceMetropolis(p, nChains=1, scale=0.05, seed=2, count_fallbacks=True)
+    mh = DeviceMetropolis(p, nChains=1, scale=0.05, seed=2, count_fallbacks=True, integrate_corner=True)
     theta = p.theta_star.copy()
     mh.set_theta(theta)
     got = int(mh.n_fallback.cpu().numpy()[0])
@@ -134,6 +134,36 @@ def test_fallback_counter_matches_calc_T(problem, orc, gpu):
     _, nfb, _ = orc.calc_T(p.space.tessellation.geom(), p.space.tessellation.dense_grid_img(), T, sA,
                            p.params["dt"], p.params["nTimeSteps"], p.params["nStepsODEsolver"], probe=True)
     assert abs(got - int(nfb.sum())) <= 0.01 * nfb.sum() + 2
+
+
+@pytest.mark.parametrize("spec", [1, 8])
+def test_corner_shortcut_is_the_same_chain(problem, gpu, spec):
+    """The domain-corner pixel left out where the bound on its path fixes its
+    sample to src(0, 0): theta, the ll trace, the accept count and the step
+    counter are bit-identical to the chain that integrates it, and the
+    shortcut ran (the corner's rejected Trel proposals, ~1 per step, are
+    missing from the fallback count)."""
+    import torch
+    from cpabdiffeo_amd.registration import DeviceMetropolis
+    say("corner shortcut", spec)
+    p = problem
+    out = []
+    for integ in (True, False):
+        mh = DeviceMetropolis(p, nChains=1, scale=0.05, seed=11, trace_len=60, count_fallbacks=True,
+                              spec_depth=spec, integrate_corner=integ)
+        mh.set_theta(p.theta_star.copy())
+        mh.run(60)
+        torch.cuda.synchronize()
+        out.append([x.cpu().numpy().copy() for x in (mh.theta, mh.trace, mh.n_accepted, mh.step_counter,
+                                                       mh.n_fallback)])
+        mh.close()
+    a, b = out
+    for i in range(4):
+        assert np.array_equal(a[i], b[i]), i
+    assert 0 < a[2][0] < 60
+    missing = int(a[4][0] - b[4][0])
+    say("corner fallbacks left out", missing)
+    assert 30 <= missing <= 61 * p.params["nTimeSteps"]  # 60 steps and the start's evaluation
 
 
 @pytest.mark.parametrize("nChains,spec,graph", [(1, 8, False), (1, 8, True), (3, 5, True), (1, 0, True)])
