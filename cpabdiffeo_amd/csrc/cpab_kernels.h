@@ -296,9 +296,10 @@ __device__ __forceinline__ bool trip(double* p, int& c, int& k, const double*& r
 // is the domain corner (0, 0): its velocity is zero up to rounding, it
 // drifts just outside the domain, and the Trel proposal leaves the cell at
 // most steps, so the lane runs up to 1 100 trips while its neighbours need
-// ~104 -- and in the registration step its slot cannot be summed before it
-// ends.  Here the lane loops on its own: its branches are uniform (one lane),
-// each trip does only its own kind of work, and the lookup is branch-free
+// ~104.  (In the registration step, where only its FP32 sample matters, a
+// bound on its path usually fixes that sample and the pixel is left out:
+// corner_sample_is_src00.)  Here the lane loops on its own: its branches are
+// uniform (one lane), each trip does only its own kind of work, and the lookup is branch-free
 // (cell2_flat, no strict-interior test).  Measured on the one-chain
 // registration: ~770 instead of ~940 cycles a trip, +5 % at 40 slots, +11 %
 // at 24.  (Holding the table row in registers took the step kernel past 64

@@ -225,8 +225,9 @@ __device__ inline void mcmc_arm_theta(const McmcDesc& D, int ch, int64_t step, i
 // (longest-processing-time first; 256 chains one step per launch: +7 %,
 // measured on MI355X).  With one pixel per lane a block's duration follows
 // its slowest pixel, and ordering by that measured no faster than the
-// identity (the corner pixel's block, the longest chain of trips, is
-// dispatched first anyway), so such launches keep the identity and skip the
+// identity (the corner pixel's block, the longest chain of trips when the
+// corner is integrated, is dispatched first anyway), so such launches keep
+// the identity and skip the
 // lookup.  Only the timing depends on the order: every block stores its
 // sums at its pixel block's index.  Above kOrderMax blocks the quadratic
 // ranking is skipped and the order is the identity.
@@ -573,7 +574,8 @@ __device__ inline bool corner_sample_is_src00(const double* Ttab, const double* 
 // One launch evaluates up to K proposals of every chain: grid (K * nBlocks,
 // nChains), or (nChains, K * nBlocks) for several chains.  Dispatch order:
 // the first pixel block of every slot (it holds the domain corner, whose
-// pixel falls back at most steps: the longest chain of trips), then the
+// pixel, when it is integrated, falls back at most steps: the longest chain
+// of trips), then the
 // slots one after another; with several chains, chain-fastest at each
 // place, so every chain's corner blocks start first.  Slot m is accepted on
 // the all-accepted path or not as soon as slots m-1 and m are summed, so a
