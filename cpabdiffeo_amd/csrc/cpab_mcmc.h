@@ -564,8 +564,10 @@ __device__ inline bool corner_sample_is_src00(const double* Ttab, const double* 
   const float v01 = W > 1 ? D.src[1] : 0.f;
   const float v10 = H > 1 ? D.src[W] : 0.f;
   const float v11 = (W > 1 && H > 1) ? D.src[W + 1] : 0.f;
+  // (a NaN or infinite neighbour would reach the sample through its tiny weight)
+  if (!(isfinite(v00) && isfinite(v01) && isfinite(v10) && isfinite(v11))) return false;
   const double m = fmax(fmax(fabs((double)v00), fabs((double)v01)), fmax(fabs((double)v10), fabs((double)v11)));
-  if (!(fabs((double)v00) >= 0x1p-126) || !(m <= 3.5e38)) return false;
+  if (!(fabs((double)v00) >= 0x1p-126)) return false;
   int e;
   (void)frexp((double)v00, &e);  // |v00| in [2^(e-1), 2^e): its FP32 ulp is 2^(e-24)
   return 4.0 * rho * m * (1.0 + 0x1p-20) <= ldexp(1.0, e - 24);

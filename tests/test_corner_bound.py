@@ -53,7 +53,7 @@ def sample_fixed(rho, img, inc):
     if not (rho <= 2.0 ** -27 and rho < 0.5 * inc):
         return False
     v = [abs(float(img[0, 0])), abs(float(img[0, 1])), abs(float(img[1, 0])), abs(float(img[1, 1]))]
-    if not v[0] >= 2.0 ** -126:
+    if not (np.isfinite(v).all() and v[0] >= 2.0 ** -126):
         return False
     e = np.frexp(v[0])[1]
     return 4.0 * rho * max(v) * (1.0 + 2.0 ** -20) <= 2.0 ** (e - 24)
@@ -97,3 +97,25 @@ def test_corner_bound_grows_with_the_corner_velocity():
         rho = corner_bound(T, A, 0.001, 100, 10)
         assert rho >= eps * 1.0
         assert rho <= eps * 1.1 + 1e-300
+
+
+def test_sample_claim_edges(orc):
+    """The sample near the corner in each quadrant is src(0, 0), and the
+    guard refuses what would break it: a NaN neighbour reaches the sample
+    through its tiny weight, and a zero corner value leaves the tiny terms."""
+    n = 64
+    img = configs.blob_image(n, 40, seed=3)
+    pts = np.zeros((n * n, 2))
+    for x, y in ((1e-12, 1e-12), (-1e-12, 1e-12), (1e-12, -1e-12), (-1e-12, -1e-12), (0.0, -0.0)):
+        pts[0] = (x, y)
+        assert orc.remap2d(pts, img)[0, 0] == img[0, 0]
+    assert sample_fixed(1e-12, img, 16.0)
+    bad = img.copy()
+    bad[0, 1] = np.nan
+    pts[0] = (1e-12, 1e-12)
+    assert np.isnan(orc.remap2d(pts, bad)[0, 0])
+    assert not sample_fixed(1e-12, bad, 16.0)
+    zero = img.copy()
+    zero[0, 0] = 0.0
+    assert orc.remap2d(pts, zero)[0, 0] != 0.0
+    assert not sample_fixed(1e-12, zero, 16.0)
