@@ -649,10 +649,12 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
     const double h = D.h;
     if (begin < nPts) {
       // The domain corner's sample, fixed by a bound on its path: its lane
-      // takes no trips (one pixel per lane only, so every lane's sum keeps
-      // its terms and order; its rejected Trel proposals are not counted).
-      const bool corner_fixed =
-          begin == 0 && D.ptsPerWave == 64 && !D.integrate_corner && corner_sample_is_src00<TESS>(Ttab, Atab, D);
+      // takes no trips (its rejected Trel proposals are then not counted).
+      // Its lane takes no further pixel either way, so the other lanes take
+      // the same pixels, and every lane's sum the same terms in the same
+      // order, whether or not the corner is integrated.
+      const bool corner_fixed = begin == 0 && !D.integrate_corner && corner_sample_is_src00<TESS>(Ttab, Atab, D);
+      bool corner_lane = false;
       // Abort checkpoints: with one pixel per lane every lane runs steps 0..nT
       // once, so the wave can stop at fixed steps when its slot lies above a
       // rejection found meanwhile (no cost per trip: the trip loop's bound
@@ -674,9 +676,9 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
           pxw += 1.0;
           active = false;
         }
-        const uint64_t idle = ballot(!active);
+        const uint64_t idle = ballot(!active && !corner_lane);
         if (idle != 0 && next < end) {
-          if (!active) {
+          if (!active && !corner_lane) {
             const int64_t cand = next + rank_in(idle);
             if (cand < end) {
               my = cand;
@@ -694,11 +696,14 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
               row = Ttab + mul24(c, 6);
               t = 0;
               k = 0;
-              if (corner_fixed && pix == 0) {  // its sample: src(0, 0) (corner_sample_is_src00)
-                const double e = (double)D.src[0] - (double)D.dst[0];
-                acc += e * e;
-                pxw += 1.0;
-                active = false;
+              if (pix == 0) {
+                corner_lane = true;
+                if (corner_fixed) {  // its sample: src(0, 0) (corner_sample_is_src00)
+                  const double e = (double)D.src[0] - (double)D.dst[0];
+                  acc += e * e;
+                  pxw += 1.0;
+                  active = false;
+                }
               }
             }
           }

@@ -347,11 +347,11 @@ typedef struct {
   int integrate_corner; /* 0: the pixel at the domain corner (0, 0) is left out of the
                            integration when a bound on its path, from the tables of the
                            corner's cells, shows that its FP32 bilinear sample is
-                           src(0, 0) bit for bit (the log-likelihood is then the same;
-                           one pixel per lane only; that pixel's rejected Trel proposals
-                           are not counted in n_fallback and stats).  Nonzero: always
-                           integrate it (no reference counterpart: the reference
-                           integrates every pixel). */
+                           src(0, 0) bit for bit (the log-likelihood is then the same: the
+                           corner's lane takes no other pixel either way; that pixel's
+                           rejected Trel proposals are not counted in n_fallback and
+                           stats).  Nonzero: always integrate it (no reference
+                           counterpart: the reference integrates every pixel). */
 } cpab_reg2d_desc;
 
 typedef struct cpab_reg2d cpab_reg2d;
