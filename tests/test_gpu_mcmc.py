@@ -136,8 +136,8 @@ def test_fallback_counter_matches_calc_T(problem, orc, gpu):
     assert abs(got - int(nfb.sum())) <= 0.01 * nfb.sum() + 2
 
 
-@pytest.mark.parametrize("spec", [1, 8])
-def test_corner_shortcut_is_the_same_chain(problem, gpu, spec):
+@pytest.mark.parametrize("spec,ppw", [(1, 0), (8, 0), (1, 256)])
+def test_corner_shortcut_is_the_same_chain(problem, gpu, spec, ppw):
     """The domain-corner pixel left out where the bound on its path fixes its
     sample to src(0, 0): theta, the ll trace, the accept count and the step
     counter are bit-identical to the chain that integrates it, and the
@@ -145,12 +145,12 @@ def test_corner_shortcut_is_the_same_chain(problem, gpu, spec):
     missing from the fallback count)."""
     import torch
     from cpabdiffeo_amd.registration import DeviceMetropolis
-    say("corner shortcut", spec)
+    say("corner shortcut", spec, ppw)
     p = problem
     out = []
     for integ in (True, False):
         mh = DeviceMetropolis(p, nChains=1, scale=0.05, seed=11, trace_len=60, count_fallbacks=True,
-                              spec_depth=spec, integrate_corner=integ)
+                              spec_depth=spec, pts_per_wave=ppw, integrate_corner=integ)
         mh.set_theta(p.theta_star.copy())
         mh.run(60)
         torch.cuda.synchronize()
