@@ -77,6 +77,7 @@ struct AdjDesc {
   double h;
   int nT, nS;
   int tile8;             // MODE 1: a wave's 64 pixels are an 8 x 8 tile (W and H multiples of 8)
+  int inst_major;        // grid (nInst, blocks): every instance's first block (the domain corner) starts first
   Geom g;
 };
 
@@ -167,7 +168,13 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
   extern __shared__ double lds[];
   __shared__ double red[4];
   constexpr int NG = FWD ? 2 : 1;
-  const int inst = blockIdx.y;
+  // grid (blocks, nInst), or (nInst, blocks) with inst_major: the hardware
+  // dispatches x fastest, so the instances' first blocks, which hold the
+  // domain corner (a pixel that can redo most of its steps: the longest
+  // chain of trips), then start before any instance's other blocks
+  const int inst = D.inst_major ? (int)blockIdx.x : (int)blockIdx.y;
+  const int64_t blk = D.inst_major ? (int64_t)blockIdx.y : (int64_t)blockIdx.x;
+  const int64_t nblk = D.inst_major ? (int64_t)gridDim.y : (int64_t)gridDim.x;
   const int nC = D.nC, NT = nC * 6;
   const int nW = blockDim.x >> 6, wave = threadIdx.x >> 6;
   double* At = lds;
@@ -185,7 +192,7 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
   const Geom& g = D.g;
   const double h = D.h;
   const int nT = D.nT, nS = D.nS;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t idx = blk * blockDim.x + threadIdx.x;
   const int64_t nPts = D.nPts;
   double loss = 0.0;
   int nredo = 0;  // time steps redone by RK2 (every step of calc_T_simple)
@@ -399,7 +406,7 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
   const double nr = block_sum((double)nredo, red);
   __syncthreads();
   // the block's tables: its waves' tables summed in wave order
-  double* out = D.partial + ((int64_t)inst * gridDim.x + blockIdx.x) * (NG * NT + 2);
+  double* out = D.partial + ((int64_t)inst * nblk + blk) * (NG * NT + 2);
   for (int i = threadIdx.x; i < NG * NT; i += blockDim.x) {
     double s = 0.0;
     for (int w = 0; w < nW; w++) s += Gall[(size_t)w * NG * NT + i];

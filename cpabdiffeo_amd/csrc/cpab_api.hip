@@ -685,7 +685,10 @@ int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts
   D.nS = nS;
   D.tile8 = (mode == 1 && H % 8 == 0 && W % 8 == 0) ? 1 : 0;
   D.g = g;
-  const dim3 grid((unsigned)nbx, (unsigned)nInst);
+  // instances' first blocks (the domain corner's long chain) first, while
+  // the blocks per instance fit the grid's y dimension
+  D.inst_major = nbx <= 65535 ? 1 : 0;
+  const dim3 grid = D.inst_major ? dim3((unsigned)nInst, (unsigned)nbx) : dim3((unsigned)nbx, (unsigned)nInst);
   hipEvent_t a = nullptr, b = nullptr;
   if ((r = prof_begin(ctx, &a, &b))) return r;
 #define ADJ_LAUNCH4(T_, P_, M_, F_)                                        \
