@@ -342,7 +342,8 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
       // move, so a redone step costs its wave one pass of its nS substeps
       // shared by several lanes rather than one pass per lane and step.
       // Every choice is a ballot of the data: the sums keep a fixed order.
-      constexpr int kRedoBatch = 8;
+      // (12 lanes measured best on MI355X: 6, 8 and 16 within 4 %.)
+      constexpr int kRedoBatch = 12;
       int k = nT - 1;  // this lane's step (-1: done)
       double q[2] = {ck[((int64_t)k * nPts + idx) * 2], ck[((int64_t)k * nPts + idx) * 2 + 1]};
       bits = rb[(int64_t)(k >> 6) * nPts + idx];
