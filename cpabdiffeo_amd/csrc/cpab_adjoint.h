@@ -422,24 +422,35 @@ __global__ void __launch_bounds__(256, 4) k_adjoint2d(AdjDesc D) {
 // Sum the per-block partials of every instance in a fixed order:
 // gradA (nInst, nC*6), gradT (nInst, nC*6; calc_T map only), loss (nInst,)
 // and redone steps (nInst,), the last two optional.
+// Grid (ceil((NG*NT + 2) / blockDim.x), nInst): one thread per entry, its
+// blocks' partials loaded eight at a time ahead of the dependent sum, which
+// adds them in block order.
 __global__ void k_adj_reduce(const double* __restrict__ partial, int nBlocks, int NT, int NG,
                              double* __restrict__ gradA, double* __restrict__ gradT, double* __restrict__ loss,
                              double* __restrict__ nredo) {
-  const int inst = blockIdx.x;
+  const int inst = blockIdx.y;
   const int stride = NG * NT + 2;
-  const double* pi = partial + (int64_t)inst * nBlocks * stride;
-  for (int e = threadIdx.x; e < stride; e += blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < nBlocks; b++) s += pi[(int64_t)b * stride + e];
-    if (e < NT) {
-      gradA[(int64_t)inst * NT + e] = s;
-    } else if (e < NG * NT) {
-      gradT[(int64_t)inst * NT + e - NT] = s;
-    } else if (e == NG * NT) {
-      if (loss) loss[inst] = s;
-    } else if (nredo) {
-      nredo[inst] = s;
-    }
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= stride) return;
+  const double* pi = partial + (int64_t)inst * nBlocks * stride + e;
+  double s = 0.0;
+  int b = 0;
+  for (; b + 8 <= nBlocks; b += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = pi[(int64_t)(b + u) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += v[u];
+  }
+  for (; b < nBlocks; b++) s += pi[(int64_t)b * stride];
+  if (e < NT) {
+    gradA[(int64_t)inst * NT + e] = s;
+  } else if (e < NG * NT) {
+    gradT[(int64_t)inst * NT + e - NT] = s;
+  } else if (e == NG * NT) {
+    if (loss) loss[inst] = s;
+  } else if (nredo) {
+    nredo[inst] = s;
   }
 }
 

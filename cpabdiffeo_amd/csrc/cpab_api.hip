@@ -726,8 +726,8 @@ int adjoint2d(cpab_ctx* ctx, int mode, int tess, int nC, int nInst, int64_t nPts
   HIP_OK(ctx, hipGetLastError());
   if ((r = prof_end(ctx, a, b))) return r;
   double* gradT = fwd ? ctx->ws + ck + pa : nullptr;
-  hipLaunchKernelGGL(k_adj_reduce, dim3(nInst), dim3(256), 0, ctx->stream, D.partial, (int)nbx, NT, NG, gradA, gradT,
-                     loss, nredo);
+  hipLaunchKernelGGL(k_adj_reduce, dim3((unsigned)((NG * NT + 2 + 63) / 64), (unsigned)nInst), dim3(64), 0, ctx->stream,
+                     D.partial, (int)nbx, NT, NG, gradA, gradT, loss, nredo);
   if (fwd) {
     const int64_t n = (int64_t)nInst * nC;
     hipLaunchKernelGGL(k_adj_expm, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, ctx->stream, As, gradT, gradA, nC,
