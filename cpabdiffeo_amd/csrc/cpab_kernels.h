@@ -158,7 +158,9 @@ __global__ void k_prep(const double* __restrict__ As, int nC, int sign, double d
 
 // Avees = B theta for m thetas (CpaSpace.theta2Avees, cpab/cpaNd/_CpaSpace.py:366-376):
 // one lane per (theta, row), the row's dot product summed in index order, so
-// a theta gives the same Avees whatever else is in the batch.
+// a theta gives the same Avees whatever else is in the batch.  The products'
+// operands are loaded eight at a time ahead of the dependent sum (one load
+// at a time left the call latency bound: ~30 us for a 3840-row table).
 __global__ void k_theta2avees(const double* __restrict__ B, const double* __restrict__ theta,
                               double* __restrict__ Av, int D, int d, int m) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -168,7 +170,18 @@ __global__ void k_theta2avees(const double* __restrict__ B, const double* __rest
   const double* row = B + (int64_t)r * d;
   const double* th = theta + i * d;
   double acc = 0.0;
-  for (int j = 0; j < d; j++) acc += row[j] * th[j];
+  int j = 0;
+  for (; j + 8 <= d; j += 8) {
+    double a[8], t[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      a[u] = row[j + u];
+      t[u] = th[j + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc += a[u] * t[u];
+  }
+  for (; j < d; j++) acc += row[j] * th[j];
   Av[idx] = acc;
 }
 
