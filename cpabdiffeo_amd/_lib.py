@@ -28,6 +28,7 @@ SIGNATURES = {
     "cpab_prep_trels": [_p, _i, _i, _p, _i, _d, _p, _p],
     "cpab_expm": [_p, _i, _i, _p, _p, _i],
     "cpab_calc_T": [_p, _i, _i, _i, _p, _p, _p, _p, _d, _i, _i, _i64] + _GEO + [_p],
+    "cpab_calc_T_prep": [_p, _i, _i, _i, _p, _p, _p, _i, _d, _i, _i, _i64] + _GEO + [_p, _p, _p],
     "cpab_calc_T_simple": [_p, _i, _i, _i, _p, _p, _p, _d, _i, _i, _i64] + _GEO,
     "cpab_calc_trajectory": [_p, _i, _i, _i, _p, _p, _p, _p, _d, _i, _i, _i64] + _GEO,
     "cpab_calc_grad_theta": [_p, _i, _i, _i, _p, _p, _p, _p, _p, _i, _d, _i, _i, _i64] + _GEO,

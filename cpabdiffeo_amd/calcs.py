@@ -101,6 +101,31 @@ class Calcs:
                  int(p0.shape[0]), *self.parse_nCs_and_incs(dim, incs, nCs), nfb)
         return pts_at_T
 
+    def calc_T_prep(self, As_vectorized, mysign, pts0, dt, nTimeSteps, nStepsOdeSolver, pts_at_T, dim_domain,
+                    dim_range, nCs, incs, signedAs_out, Tlocals_out, n_fallback=None):
+        """prepare_signedSqAs_and_Tlocals_for_gpu + calc_T (cpab/cpaNd/_CpaCalcs.py:227-313, 376-558)
+        in one launch: each block forms the signed table and the Trels from the raw As itself;
+        the tables are also written to signedAs_out / Tlocals_out.  The same bits as the two
+        calls."""
+        if pts_at_T is None:
+            raise ValueError
+        if int(mysign) not in (1, -1):
+            raise ValueError(mysign)
+        dim = dim_domain
+        A, sA, T = self._tables(dim, As_vectorized, signedAs_out, Tlocals_out)
+        p0 = self._pts(pts0, dim)
+        out = self._pts(pts_at_T, dim)
+        if tuple(out.shape) != tuple(p0.shape):
+            raise ValueError(tuple(out.shape), tuple(p0.shape))
+        nfb = 0
+        if n_fallback is not None:
+            nfb = dev_tensor(n_fallback, "int32", (p0.shape[0],)).data_ptr()
+        ctx = context_for(p0)
+        ctx.call("cpab_calc_T_prep", dim, self.tess_type, self.nCells, p0.data_ptr(), out.data_ptr(), A.data_ptr(),
+                 int(mysign), float(dt), int(nTimeSteps), int(nStepsOdeSolver), int(p0.shape[0]),
+                 *self.parse_nCs_and_incs(dim, incs, nCs), sA.data_ptr(), T.data_ptr(), nfb)
+        return pts_at_T
+
     def calc_T_simple(self, xmins, xmaxs, As_vectorized, pts0, dt, nTimeSteps, nStepsOdeSolver,
                       pts_at_T, dim_domain, dim_range, nCs, incs, timer=None):
         """cpab/gpu/Calcs.py:339-450."""

@@ -207,13 +207,13 @@ int prof_end(cpab_ctx* ctx, hipEvent_t a, hipEvent_t b, int launches = 1) {
 template <int DIM, int TESS, bool POW2, bool LDS>
 int launch_integrate_t(cpab_ctx* ctx, int mode, const Shape& s, const double* pos0, double* out,
                        const double* Trels, const double* As, double h, int nT, int nS, int64_t nPts,
-                       const Geom& g, int nC, int32_t* nfb) {
+                       const Geom& g, int nC, int32_t* nfb, const PrepSrc& ps) {
   if constexpr (DIM == 2 && TESS == TESS_I && POW2) {
     if (mode == 0 && s.fast) {
       auto k = k_integrate<DIM, TESS, POW2, LDS, 0, true>;
       allow_lds(k, s.lds);
       hipLaunchKernelGGL(k, dim3((unsigned)s.grid), dim3(s.block), s.lds, ctx->stream, pos0, out, Trels, As, h, nT,
-                         nS, nPts, g, nC, s.ptsPerWave, nfb);
+                         nS, nPts, g, nC, s.ptsPerWave, nfb, ps);
       HIP_OK(ctx, hipGetLastError());
       return CPAB_OK;
     }
@@ -222,12 +222,12 @@ int launch_integrate_t(cpab_ctx* ctx, int mode, const Shape& s, const double* po
     auto k = k_integrate<DIM, TESS, POW2, LDS, 0>;
     allow_lds(k, s.lds);
     hipLaunchKernelGGL(k, dim3((unsigned)s.grid), dim3(s.block), s.lds, ctx->stream, pos0, out, Trels, As, h, nT, nS,
-                       nPts, g, nC, s.ptsPerWave, nfb);
+                       nPts, g, nC, s.ptsPerWave, nfb, ps);
   } else {
     auto k = k_integrate<DIM, TESS, POW2, LDS, 1>;
     allow_lds(k, s.lds);
     hipLaunchKernelGGL(k, dim3((unsigned)s.grid), dim3(s.block), s.lds, ctx->stream, pos0, out, Trels, As, h, nT, nS,
-                       nPts, g, nC, s.ptsPerWave, nfb);
+                       nPts, g, nC, s.ptsPerWave, nfb, ps);
   }
   HIP_OK(ctx, hipGetLastError());
   return CPAB_OK;
@@ -236,25 +236,26 @@ int launch_integrate_t(cpab_ctx* ctx, int mode, const Shape& s, const double* po
 template <int DIM, int TESS>
 int launch_integrate_d(cpab_ctx* ctx, int mode, bool pow2, const Shape& s, const double* pos0, double* out,
                        const double* Trels, const double* As, double h, int nT, int nS, int64_t nPts,
-                       const Geom& g, int nC, int32_t* nfb) {
+                       const Geom& g, int nC, int32_t* nfb, const PrepSrc& ps) {
   if (pow2) {
-    if (s.use_lds) return launch_integrate_t<DIM, TESS, true, true>(ctx, mode, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
-    return launch_integrate_t<DIM, TESS, true, false>(ctx, mode, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
+    if (s.use_lds) return launch_integrate_t<DIM, TESS, true, true>(ctx, mode, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb, ps);
+    return launch_integrate_t<DIM, TESS, true, false>(ctx, mode, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb, ps);
   }
-  if (s.use_lds) return launch_integrate_t<DIM, TESS, false, true>(ctx, mode, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
-  return launch_integrate_t<DIM, TESS, false, false>(ctx, mode, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
+  if (s.use_lds) return launch_integrate_t<DIM, TESS, false, true>(ctx, mode, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb, ps);
+  return launch_integrate_t<DIM, TESS, false, false>(ctx, mode, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb, ps);
 }
 
+// ps.raw != nullptr (the prep fused into the launch) only with s.use_lds.
 int launch_integrate(cpab_ctx* ctx, int mode, int dim, int tess, bool pow2, const Shape& s, const double* pos0,
                      double* out, const double* Trels, const double* As, double h, int nT, int nS, int64_t nPts,
-                     const Geom& g, int nC, int32_t* nfb) {
-  if (dim == 1) return launch_integrate_d<1, TESS_II>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
+                     const Geom& g, int nC, int32_t* nfb, const PrepSrc& ps) {
+  if (dim == 1) return launch_integrate_d<1, TESS_II>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb, ps);
   if (dim == 2) {
-    if (tess == TESS_I) return launch_integrate_d<2, TESS_I>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
-    return launch_integrate_d<2, TESS_II>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
+    if (tess == TESS_I) return launch_integrate_d<2, TESS_I>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb, ps);
+    return launch_integrate_d<2, TESS_II>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb, ps);
   }
-  if (tess == TESS_I) return launch_integrate_d<3, TESS_I>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
-  return launch_integrate_d<3, TESS_II>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb);
+  if (tess == TESS_I) return launch_integrate_d<3, TESS_I>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb, ps);
+  return launch_integrate_d<3, TESS_II>(ctx, mode, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb, ps);
 }
 
 // Generic 3-way dispatch for the simple per-point kernels.
@@ -393,7 +394,8 @@ int cpab_expm(cpab_ctx* ctx, int dim, int nC, const double* As, double* Ts, int 
 
 static int integrate_common(cpab_ctx* ctx, int mode, int dim, int tess, int nC, const double* pos0, double* out,
                             const double* Trels, const double* As, double dt, int nT, int nS, int64_t nPts,
-                            int nC0, int nC1, int nC2, double ix, double iy, double iz, int32_t* nfb) {
+                            int nC0, int nC1, int nC2, double ix, double iy, double iz, int32_t* nfb,
+                            const double* raw = nullptr, int sign = 1) {
   if (!ctx) return CPAB_ERR_ARG;
   Geom g;
   bool pow2;
@@ -412,9 +414,18 @@ static int integrate_common(cpab_ctx* ctx, int mode, int dim, int tess, int nC, 
   if (s.grid > 0x7fffffffLL) return fail(ctx, CPAB_ERR_ARG, "too many points for one launch");
   // h = dt/double(nStepsOdeSolver) (calc_2Dto2D.cu:383)
   const double h = dt / (double)nS;
+  // the prep fused into the launch when the tables fit LDS; else its own launch first
+  PrepSrc ps{nullptr, nullptr, nullptr, 0.0, 1};
+  if (raw) {
+    if (s.use_lds) {
+      ps = PrepSrc{raw, const_cast<double*>(As), const_cast<double*>(Trels), dt, sign};
+    } else if ((r = cpab_prep_trels(ctx, dim, nC, raw, sign, dt, const_cast<double*>(As), const_cast<double*>(Trels)))) {
+      return r;
+    }
+  }
   hipEvent_t a = nullptr, b = nullptr;
   if ((r = prof_begin(ctx, &a, &b))) return r;
-  if ((r = launch_integrate(ctx, mode, dim, tess, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb))) return r;
+  if ((r = launch_integrate(ctx, mode, dim, tess, pow2, s, pos0, out, Trels, As, h, nT, nS, nPts, g, nC, nfb, ps))) return r;
   return prof_end(ctx, a, b);
 }
 
@@ -423,6 +434,21 @@ int cpab_calc_T(cpab_ctx* ctx, int dim, int tess, int nC, const double* pos0, do
                 double iy, double iz, int32_t* n_fallback) {
   return integrate_common(ctx, 0, dim, tess, nC, pos0, pos, Trels, As, dt, nT, nS, nPts, nC0, nC1, nC2, ix, iy, iz,
                           n_fallback);
+}
+
+int cpab_calc_T_prep(cpab_ctx* ctx, int dim, int tess, int nC, const double* pos0, double* pos, const double* As,
+                     int sign, double dt, int nT, int nS, int64_t nPts, int nC0, int nC1, int nC2, double ix,
+                     double iy, double iz, double* signedAs_out, double* Trels_out, int32_t* n_fallback) {
+  if (!ctx) return CPAB_ERR_ARG;
+  if (sign != 1 && sign != -1) return fail(ctx, CPAB_ERR_ARG, "sign must be +1 or -1");
+  if (!As || !signedAs_out || !Trels_out) return fail(ctx, CPAB_ERR_ARG, "null table");
+  if (nC < 1) return fail(ctx, CPAB_ERR_ARG, "nC must be >= 1");
+  const size_t nb = (size_t)nC * dim * (dim + 1) * sizeof(double);
+  if (overlap(As, nb, Trels_out, nb) || overlap(As, nb, signedAs_out, nb) || overlap(signedAs_out, nb, Trels_out, nb))
+    return fail(ctx, CPAB_ERR_ALIAS, "the table outputs overlap each other or As");
+  if (nPts == 0) return cpab_prep_trels(ctx, dim, nC, As, sign, dt, signedAs_out, Trels_out);  // the tables still
+  return integrate_common(ctx, 0, dim, tess, nC, pos0, pos, Trels_out, signedAs_out, dt, nT, nS, nPts, nC0, nC1, nC2,
+                          ix, iy, iz, n_fallback, As, sign);
 }
 
 int cpab_calc_trajectory(cpab_ctx* ctx, int dim, int tess, int nC, const double* pos0, double* traj,

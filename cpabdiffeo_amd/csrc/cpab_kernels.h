@@ -243,6 +243,41 @@ __device__ __forceinline__ void stage_tables(double* lds, const double* __restri
   __syncthreads();
 }
 
+// The per-cell prep fused into a launch (cpab_calc_T_prep): the raw Avees
+// table with k_prep's sign and dt.  Every block forms the tables itself
+// (prep_cell, the same operations as k_prep) into LDS; block 0 also writes
+// them out as cpab_prep_trels would.  raw == nullptr: the tables come ready.
+struct PrepSrc {
+  const double* raw;
+  double* sA_out;
+  double* T_out;
+  double dt;
+  int sign;
+};
+
+template <int DIM>
+__device__ __forceinline__ void stage_tables_prep(double* lds, const PrepSrc& ps, int nC) {
+  constexpr int NE = DIM * (DIM + 1);
+  const int n = nC * NE;
+  for (int c = threadIdx.x; c < nC; c += blockDim.x) {
+    double sA[NE], T[NE];
+    prep_cell<DIM>(ps.raw + (int64_t)c * NE, ps.sign, ps.dt, sA, T, 12);
+#pragma unroll
+    for (int e = 0; e < NE; e++) {
+      lds[c * NE + e] = T[e];
+      lds[n + c * NE + e] = sA[e];
+    }
+    if (blockIdx.x == 0) {
+#pragma unroll
+      for (int e = 0; e < NE; e++) {
+        ps.T_out[(int64_t)c * NE + e] = T[e];
+        ps.sA_out[(int64_t)c * NE + e] = sA[e];
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // ----------------------------------------------------------------- trip
 // One unit of the reference's step schedule for one point (see k_integrate):
 //   k == 0: y = Trel_c [p;1]; if cell(y) == c accept, else start the RK2 redo
@@ -449,13 +484,14 @@ __global__ void __launch_bounds__(1024)
     k_integrate(const double* __restrict__ pos0, double* __restrict__ out,
                 const double* __restrict__ Trels, const double* __restrict__ As, double h,
                 int nT, int nS, int64_t nPts, Geom g, int nC, int ptsPerWave,
-                int32_t* __restrict__ nfb_out) {
+                int32_t* __restrict__ nfb_out, PrepSrc ps) {
   constexpr int NE = DIM * (DIM + 1);
   extern __shared__ double lds[];
   const double* Ttab = Trels;
   const double* Atab = As;
   if (LDS) {
-    stage_tables<NE>(lds, Trels, As, nC);
+    if (ps.raw) stage_tables_prep<DIM>(lds, ps, nC);  // (a kernel argument: uniform)
+    else stage_tables<NE>(lds, Trels, As, nC);
     Ttab = lds;
     Atab = lds + nC * NE;
   }

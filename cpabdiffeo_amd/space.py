@@ -390,10 +390,12 @@ class CpaSpace:
         dt = params_flow_int["dt"]
         nT = params_flow_int["nTimeSteps"]
         nS = params_flow_int["nStepsODEsolver"]
-        sA, T = self.prepare_signed_As_and_Tlocals(mysign, dt, pat)
-        self._gpu_calcs.calc_T(None, None, T, sA, pts, dt, nT, nS, out, self.dim_domain,
-                               self.dim_range, self.nCs, self.incs, timer=timer,
-                               n_fallback=n_fallback)
+        # prepare_signed_As_and_Tlocals and calc_T in one launch (the same tables and bits)
+        pat = self.pat if pat is None else pat
+        g = self._dev()
+        self._gpu_calcs.calc_T_prep(pat.device_table(), int(mysign), pts, dt, nT, nS, out, self.dim_domain,
+                                    self.dim_range, self.nCs, self.incs, g["signed_As"], g["Tlocals"],
+                                    n_fallback=n_fallback)
         return out
 
     def calc_T_fwd(self, pat=None, pts=None, do_checks=True, out=None, **params_flow_int):

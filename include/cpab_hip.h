@@ -78,6 +78,20 @@ int cpab_calc_T(cpab_ctx *ctx, int dim, int tess, int nC, const double *pos0, do
                 int64_t nPts, int nC0, int nC1, int nC2, double inc_x, double inc_y,
                 double inc_z, int32_t *n_fallback);
 
+/* calc_T with the per-cell prep in the same launch: prepare_signedSqAs_and_
+ * Tlocals_for_gpu followed by calc_T (cpab/cpaNd/_CpaCalcs.py:227-313,
+ * 376-558; the kernels above).  As is the raw Avees table (nC x dim(dim+1));
+ * every block forms sign * As and Trels = expm(dt * sign * As) itself (the
+ * same operations as cpab_prep_trels) and block 0 writes them to
+ * signedAs_out and Trels_out (the space's persistent tables, both
+ * required), so results, tables and n_fallback are the bits of
+ * cpab_prep_trels + cpab_calc_T.  When the tables do not fit LDS the prep
+ * runs as its own launch first. */
+int cpab_calc_T_prep(cpab_ctx *ctx, int dim, int tess, int nC, const double *pos0, double *pos,
+                     const double *As, int sign, double dt, int nTimeSteps, int nStepsODE,
+                     int64_t nPts, int nC0, int nC1, int nC2, double inc_x, double inc_y,
+                     double inc_z, double *signedAs_out, double *Trels_out, int32_t *n_fallback);
+
 /* calc_T_simple: Calcs.calc_T_simple (cpab/gpu/Calcs.py:339-450), kernel
  * cpab/gpu/calc_2Dto2D.cu:412-445 (nStepsODE*nTimeSteps RK2 substeps). */
 int cpab_calc_T_simple(cpab_ctx *ctx, int dim, int tess, int nC, const double *pos0, double *pos,

@@ -382,3 +382,43 @@ def test_calc_T_outside_wall_and_special_points(gpu, orc, n):
         got = host(out)
         assert np.array_equal(got, want, equal_nan=True), (sign, np.nanmax(np.abs(got - want)))
         assert np.array_equal(host(nfb), nfb_w)
+
+
+@pytest.mark.parametrize("name,lo,hi,nCs,tess", INTEG + [("2D-I-32x32-no-LDS", [0, 0], [1, 1], (32, 32), "I")])
+def test_calc_T_prep_equals_prep_then_calc_T(gpu, name, lo, hi, nCs, tess):
+    """cpab_calc_T_prep (the per-cell prep formed by every block of the
+    integration launch, or its own launch when the tables do not fit LDS)
+    against cpab_prep_trels + cpab_calc_T: end points, fallback counts and
+    the tables it writes, bit for bit, both signs, one and several points per
+    lane."""
+    import ctypes
+    from cpabdiffeo_amd._lib import lib
+    torch = gpu
+    say("calc_T_prep", name)
+    sp = space_for(lo, hi, nCs, tess)
+    dim, ne = sp.dim_domain, sp.dim_domain * (sp.dim_domain + 1)
+    Av = dev(torch, random_field(sp, 60).reshape(sp.nC, ne))
+    calcs = sp._gpu_calcs
+    for n in (5000, 300000):
+        pts = dev(torch, np.random.default_rng(61).uniform(0, 1, (n, dim)) * sp.XMAXS)
+        for sign in (1, -1):
+            sA = torch.empty((sp.nC, ne), dtype=torch.float64, device="cuda")
+            T = torch.empty_like(sA)
+            gpu_prep_trels(dim, Av, sign, 0.01, sA, T)
+            want = torch.zeros_like(pts)
+            nfb_w = torch.zeros(n, dtype=torch.int32, device="cuda")
+            calcs.calc_T(None, None, T, sA, pts, 0.01, 100, 10, want, dim, dim, sp.nCs, sp.incs, n_fallback=nfb_w)
+            sA2 = torch.full_like(sA, float("nan"))
+            T2 = torch.full_like(sA, float("nan"))
+            got = torch.zeros_like(pts)
+            nfb = torch.zeros(n, dtype=torch.int32, device="cuda")
+            calcs.calc_T_prep(Av, sign, pts, 0.01, 100, 10, got, dim, dim, sp.nCs, sp.incs, sA2, T2, n_fallback=nfb)
+            torch.cuda.synchronize()
+            assert torch.equal(got, want), (name, n, sign)
+            assert torch.equal(nfb, nfb_w)
+            assert torch.equal(sA2, sA) and torch.equal(T2, T)
+    # errors: a sign other than +-1, an output table aliasing the input
+    with pytest.raises(ValueError):
+        calcs.calc_T_prep(Av, 2, pts, 0.01, 10, 10, torch.zeros_like(pts), dim, dim, sp.nCs, sp.incs, sA2, T2)
+    with pytest.raises(ValueError):
+        calcs.calc_T_prep(Av, 1, pts, 0.01, 10, 10, torch.zeros_like(pts), dim, dim, sp.nCs, sp.incs, Av, T2)
