@@ -126,7 +126,8 @@ struct McmcDesc {
                         // pixels integrated
   int nBlocks;          // integration blocks per chain and slot
   int ptsPerWave;
-  int tile8;            // 1: pixels are taken in 8 x 8 tiles (W and H multiples of 8)
+  const uint32_t* pixmap;  // (H*W) the pixel of each work index: the domain's left column and top row
+                           // first, then the interior in 8 x 8 tiles (W and H multiples of 8) or rows
   int poll;             // abort checkpoints per pixel run (one pixel per lane only; 0: none)
   int* order;           // (nChains, nBlocks): the pixel block each dispatch position integrates,
                         // or null: dispatch position = pixel block
@@ -683,13 +684,7 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
             if (cand < end) {
               my = cand;
               active = true;
-              if (D.tile8) {
-                const int tt = (int)(my >> 6), u = (int)(my & 63), tw = D.W >> 3;
-                const int px = (tt % tw) * 8 + (u & 7), py = (tt / tw) * 8 + (u >> 3);
-                pix = (int64_t)py * D.W + px;
-              } else {
-                pix = my;
-              }
+              pix = (int64_t)D.pixmap[my];
               p[0] = (double)(pix % D.W);
               p[1] = (double)(pix / D.W);
               c = cell2<TESS, POW2, FAST>(p, g);
