@@ -1272,22 +1272,32 @@ constexpr int kAbortCheckpoints = 3;  // steps at which a one-pixel-per-lane wav
 // general path nearly every trip; in waves of their own they no longer send
 // the interior lanes of their tiles down it (one chain: +2 % measured on
 // MI355X).  The first pixel is the domain corner (0, 0).
-static std::vector<uint32_t> reg2d_pixel_order(int H, int W) {
+static std::vector<uint32_t> reg2d_pixel_order(int H, int W, int ptsPerWave) {
   std::vector<uint32_t> o;
   o.reserve((size_t)H * W);
+  // the top row, then the left column: the pixel (0, 0) first
   for (int x = 0; x < W; x++) o.push_back((uint32_t)x);
   for (int y = 1; y < H; y++) o.push_back((uint32_t)y * W);
-  if (W % 8 == 0 && H % 8 == 0) {
-    for (int ty = 0; ty < H / 8; ty++)
-      for (int tx = 0; tx < W / 8; tx++)
-        for (int v = 0; v < 8; v++)
-          for (int u = 0; u < 8; u++) {
-            const int x = tx * 8 + u, y = ty * 8 + v;
-            if (x > 0 && y > 0) o.push_back((uint32_t)y * W + x);
-          }
-  } else {
-    for (int y = 1; y < H; y++)
-      for (int x = 1; x < W; x++) o.push_back((uint32_t)y * W + x);
+  // The interior in 8 x 8 tiles from (1, 1); what is left of it at the right
+  // and the bottom (under 8 wide) in strips: the right strip row by row, the
+  // bottom strip column by column (a wave's pixels then lie in a ~7 x 9 patch).
+  const int nTx = (W - 1) / 8, nTy = (H - 1) / 8, xs = 1 + 8 * nTx, ys = 1 + 8 * nTy;
+  for (int y = 1; y < ys; y++)
+    for (int x = xs; x < W; x++) o.push_back((uint32_t)y * W + x);
+  for (int x = 1; x < W; x++)
+    for (int y = ys; y < H; y++) o.push_back((uint32_t)y * W + x);
+  // The full tiles last, as many as possible starting on a wave's first
+  // pixel (the ones that do not fit go first), so each wave's pixels are
+  // whole tiles; that needs H W to be a multiple of 64.
+  const int64_t nFull = (int64_t)nTx * nTy, nPts = (int64_t)H * W;
+  int64_t nHead = 0;
+  if (nPts % 64 == 0 && nFull > 0)
+    while (nHead < nFull && (nPts - 64 * (nFull - nHead)) % ptsPerWave != 0) nHead++;
+  for (int64_t i = 0; i < nFull; i++) {
+    const int64_t tile = i < nHead ? nFull - nHead + i : i - nHead;  // the head's tiles are the last ones
+    const int tx = (int)(tile % nTx), ty = (int)(tile / nTx);
+    for (int v = 0; v < 8; v++)
+      for (int u = 0; u < 8; u++) o.push_back((uint32_t)(1 + ty * 8 + v) * W + (uint32_t)(1 + tx * 8 + u));
   }
   return o;
 }
@@ -1581,7 +1591,7 @@ int cpab_reg2d_create(cpab_ctx* ctx, const cpab_reg2d_desc* in, cpab_reg2d** out
   r->host_step.resize(q.nChains);
   {  // no slot is aborted before the first launch; the pixel order
     const std::vector<int> none(q.nChains, kNoAbort);
-    const std::vector<uint32_t> order = reg2d_pixel_order(q.H, q.W);
+    const std::vector<uint32_t> order = reg2d_pixel_order(q.H, q.W, D.ptsPerWave);
     e = hipMemcpyAsync(D.abort_after, none.data(), sizeof(int) * q.nChains, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
       e = hipMemcpyAsync(pixmap, order.data(), sizeof(uint32_t) * order.size(), hipMemcpyHostToDevice, ctx->stream);

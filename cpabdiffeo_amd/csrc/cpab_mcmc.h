@@ -126,8 +126,8 @@ struct McmcDesc {
                         // pixels integrated
   int nBlocks;          // integration blocks per chain and slot
   int ptsPerWave;
-  const uint32_t* pixmap;  // (H*W) the pixel of each work index: the domain's left column and top row
-                           // first, then the interior in 8 x 8 tiles (W and H multiples of 8) or rows
+  const uint32_t* pixmap;  // (H*W) the pixel of each work index: the domain's top row and left
+                           // column first, then the interior's partial strips, then its 8 x 8 tiles
   int poll;             // abort checkpoints per pixel run (one pixel per lane only; 0: none)
   int* order;           // (nChains, nBlocks): the pixel block each dispatch position integrates,
                         // or null: dispatch position = pixel block
