@@ -378,6 +378,8 @@ def main():
     ap.add_argument("--no-seq", action="store_true", help="skip the one-step-per-launch comparison chain")
     ap.add_argument("--section", choices=["calc_T_fwd", "batching", "gradient_descent", "serving"],
                     help="run only this side workload and print its figures (for per-kernel profiles)")
+    ap.add_argument("--gd-steps", type=int, default=200,
+                    help="gradient-descent steps of the gradient_descent section (fewer for a counter run)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (the reference point for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -404,7 +406,7 @@ def main():
         from cpabdiffeo_amd._lib import Context
         ctx = Context.get(0)
         fn = {"calc_T_fwd": lambda: config1_extra(torch, ctx), "batching": lambda: config3_extra(torch, ctx),
-              "gradient_descent": lambda: config4_extra(torch, ctx), "serving": lambda: config5_extra(torch)}
+              "gradient_descent": lambda: config4_extra(torch, ctx, steps=args.gd_steps), "serving": lambda: config5_extra(torch)}
         print(json.dumps({args.section: fn[args.section]()}), flush=True)
         return
 
