@@ -417,13 +417,29 @@ __device__ __forceinline__ void lone_trips(double* p, int& c, int& k, int& t, in
 // Trips run in an inner loop until some lane finishes its point; only then
 // do lanes retire and refill from the wave's range (the bookkeeping costs
 // ballots and branches, so it stays out of the per-trip path).
-// One wave integrates the points [begin, end) of pos0 (indices into pos0,
-// out and nfb_out).  Wave-uniform arguments; every lane of the wave calls it.
+// Processing index i -> point index when the points form a row-major grid
+// of row length w (a multiple of 8): the first nTiled = 8 w floor(nPts / 8 w)
+// indices walk 8 x 8 tiles (64 consecutive indices one tile, the tiles of
+// each band of 8 rows in turn), the rest map to themselves.  A bijection of
+// [0, nPts) whatever w is.
+__device__ __forceinline__ int64_t tile_order(int64_t i, int w, int64_t nTiled) {
+  if (i >= nTiled) return i;
+  const int64_t t = i >> 6;
+  const int u = (int)(i & 63), tpr = w >> 3;
+  const int64_t ty = t / tpr;
+  const int tx = (int)(t - ty * tpr);
+  return (ty * 8 + (u >> 3)) * (int64_t)w + tx * 8 + (u & 7);
+}
+
+// One wave integrates the points [begin, end) of the processing order
+// (indices into pos0, out and nfb_out; with gw > 0 through tile_order).
+// Wave-uniform arguments; every lane of the wave calls it.
 template <int DIM, int TESS, bool POW2, int MODE, bool FAST = false>
 __device__ __forceinline__ void integrate_range(int64_t begin, int64_t end, const double* __restrict__ pos0,
                                                 double* __restrict__ out, const double* Ttab, const double* Atab,
                                                 double h, int nT, int nS, int64_t nPts, const Geom& g,
-                                                int32_t* __restrict__ nfb_out) {
+                                                int32_t* __restrict__ nfb_out, int gw = 0) {
+  const int64_t nTiled = gw > 0 ? nPts / (8 * (int64_t)gw) * (8 * (int64_t)gw) : 0;
   constexpr int NE = DIM * (DIM + 1);
   int64_t next = begin;  // wave-uniform cursor into [begin, end)
   double p[DIM];
@@ -442,7 +458,7 @@ __device__ __forceinline__ void integrate_range(int64_t begin, int64_t end, cons
       if (!active) {
         const int64_t cand = next + rank_in(idle);
         if (cand < end) {
-          my = cand;
+          my = gw > 0 ? tile_order(cand, gw, nTiled) : cand;
           active = true;
           load_pt<DIM>(pos0, my, p);
           c = cell_of<DIM, TESS, POW2>(p, g);
@@ -487,6 +503,26 @@ __global__ void __launch_bounds__(1024)
                 int32_t* __restrict__ nfb_out, PrepSrc ps) {
   constexpr int NE = DIM * (DIM + 1);
   extern __shared__ double lds[];
+  // calc_T of 2D points: when pos0 looks like a row-major grid, its row
+  // length W (the first index, a multiple of 8 up to 8 x blockDim.x, where y
+  // changes from point 0's), so that waves take 8 x 8 tiles: neighbouring
+  // points flow alike and cross facets at about the same steps, so a wave's
+  // lanes need similar numbers of trips.  Only the processing order depends
+  // on it (every point's result is its own), so a wrong guess costs time
+  // only.  The loads are issued before the tables are staged.
+  __shared__ int grid_w;
+  constexpr bool kDetect = MODE == 0 && DIM == 2;
+  double yc = 0.0, yp = 0.0, y0 = 0.0;
+  const int64_t ic = 8 * ((int64_t)threadIdx.x + 1);
+  if (kDetect) {
+    if (threadIdx.x == 0) grid_w = 0x7fffffff;
+    if (ic < nPts) {
+      y0 = pos0[1];
+      yc = pos0[ic * 2 + 1];
+      yp = pos0[(ic - 1) * 2 + 1];
+    }
+    __syncthreads();
+  }
   const double* Ttab = Trels;
   const double* Atab = As;
   if (LDS) {
@@ -495,11 +531,17 @@ __global__ void __launch_bounds__(1024)
     Ttab = lds;
     Atab = lds + nC * NE;
   }
+  int gw = 0;
+  if (kDetect) {
+    if (ic < nPts && yc != y0 && yp == y0) atomicMin(&grid_w, (int)ic);
+    __syncthreads();
+    gw = grid_w != 0x7fffffff && 8 * (int64_t)grid_w <= nPts ? grid_w : 0;
+  }
   const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int64_t begin = wave * (int64_t)ptsPerWave;
   if (begin >= nPts) return;
   const int64_t end = begin + ptsPerWave < nPts ? begin + ptsPerWave : nPts;
-  integrate_range<DIM, TESS, POW2, MODE, FAST>(begin, end, pos0, out, Ttab, Atab, h, nT, nS, nPts, g, nfb_out);
+  integrate_range<DIM, TESS, POW2, MODE, FAST>(begin, end, pos0, out, Ttab, Atab, h, nT, nS, nPts, g, nfb_out, gw);
 }
 
 // ---------------------------------------------------------- sensitivity
