@@ -874,7 +874,16 @@ int batch_launch(cpab_batch* b, hipStream_t s, const cpab_batch_sched& q, const 
   // Trels alone in LDS with the As read from global memory, which doubles the
   // occupancy of a 16 x 16 x 4-triangle batch, ran its best schedule 4 %
   // slower, substeps being a quarter of its trips)
-  const size_t lds = (size_t)2 * b->maxNC * NE * sizeof(double);
+  size_t lds = (size_t)2 * b->maxNC * NE * sizeof(double);
+  // 2D tiles of up to kBucketTileMax points are taken in an order by position
+  // built in LDS after the tables (measured on MI355X, 1 024 instances of
+  // uniform random points, tiles of 16 384: 54.4 -> 43 ms)
+  int sort_off = 0;
+  const size_t lds_sorted = lds + sizeof(unsigned int) * kBucketMax + sizeof(uint16_t) * (size_t)q.tile;
+  if (DIM == 2 && q.tile <= kBucketTileMax && lds_sorted <= (size_t)ctx->max_lds) {
+    sort_off = 2 * b->maxNC * NE;
+    lds = lds_sorted;
+  }
   auto k = k_integrate_batch<DIM, TESS, POW2>;
   allow_lds(k, lds);
   int per_cu = 0;
@@ -885,7 +894,7 @@ int batch_launch(cpab_batch* b, hipStream_t s, const cpab_batch_sched& q, const 
   if (grid > chunks) grid = (int)chunks;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, b->queue);
   hipLaunchKernelGGL(k, dim3(grid), dim3(q.block), lds, s, b->d_inst, b->d_tiles, b->nTiles, q.chunk, b->queue,
-                     tables, pos0, out, h, nT, nS, nfb);
+                     tables, pos0, out, h, nT, nS, nfb, sort_off);
   HIP_OK(ctx, hipGetLastError());
   return CPAB_OK;
 }

@@ -64,15 +64,80 @@ __global__ void k_batch_reset(unsigned int* queue) {
   if (threadIdx.x == 0) *queue = 0u;
 }
 
+// Processing order of a 2D tile's points by position: a counting sort of
+// the tile's point indices over an nb x nb grid of buckets on the
+// instance's domain (nb a power of two up to 32, at least 16 points a
+// bucket on average), buckets in row order, built in LDS by the whole block.
+// Neighbouring points flow alike and cross facets at about the same steps,
+// so a wave's lanes then need closer numbers of trips than for points in
+// input order.  Within a bucket the order follows the LDS atomics, which may
+// differ from run to run: that changes which lane integrates a point and
+// when, never the point's result.
+constexpr int kBucketTileMax = 16384;  // the most points of a sorted tile (16-bit indices)
+constexpr int kBucketMax = 1024;       // buckets (32 x 32)
+
+__device__ __forceinline__ int bucket_of(const double* p, double sx, double sy, int nb) {
+  // (NaN coordinates go to a border bucket: fmin/fmax return the other operand)
+  const double fx = fmax(0.0, fmin(p[0] * sx, (double)(nb - 1)));
+  const double fy = fmax(0.0, fmin(p[1] * sy, (double)(nb - 1)));
+  return (int)fy * nb + (int)fx;
+}
+
+__device__ void tile_bucket_order(const double* __restrict__ pos0, int64_t begin, int n, const Geom& g,
+                                  unsigned int* hist, uint16_t* idx) {
+  int nb = 1;
+  while (nb < 32 && 16 * (4 * nb * nb) <= n) nb *= 2;
+  const int nbk = nb * nb;
+  const double sx = (double)nb / ((double)g.nC0 * g.inc_x), sy = (double)nb / ((double)g.nC1 * g.inc_y);
+  __syncthreads();  // every wave is done with the previous tile's order
+  for (int i = threadIdx.x; i < nbk; i += blockDim.x) hist[i] = 0u;
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    double p[2];
+    load_pt<2>(pos0, begin + j, p);
+    atomicAdd(&hist[bucket_of(p, sx, sy, nb)], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {  // exclusive scan by the first wave, each lane a run of buckets
+    const int per = (nbk + 63) >> 6;
+    const int lo = (int)threadIdx.x * per;
+    const int hi = lo + per < nbk ? lo + per : nbk;
+    unsigned int sum = 0u;
+    for (int i = lo; i < hi; i++) sum += hist[i];
+    unsigned int x = sum;
+    for (int d = 1; d < 64; d <<= 1) {
+      const unsigned int y = __shfl_up(x, d, 64);
+      if ((int)threadIdx.x >= d) x += y;
+    }
+    unsigned int run = x - sum;
+    for (int i = lo; i < hi; i++) {
+      const unsigned int c = hist[i];
+      hist[i] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    double p[2];
+    load_pt<2>(pos0, begin + j, p);
+    idx[atomicAdd(&hist[bucket_of(p, sx, sy, nb)], 1u)] = (uint16_t)j;
+  }
+  __syncthreads();
+}
+
 template <int DIM, int TESS, bool POW2>
 __global__ void __launch_bounds__(1024)
     k_integrate_batch(const BatchInst* __restrict__ insts, const BatchTile* __restrict__ tiles, int nTiles, int chunk,
                       unsigned int* __restrict__ queue, const double* __restrict__ tables,
                       const double* __restrict__ pos0, double* __restrict__ out, double h, int nT, int nS,
-                      int32_t* __restrict__ nfb_out) {
+                      int32_t* __restrict__ nfb_out, int sort_off) {
   constexpr int NE = DIM * (DIM + 1);
   extern __shared__ double lds[];
   __shared__ int s_t0;
+  // sort_off > 0 (2D): the tiles' processing order by position, in LDS
+  // after the tables (sort_off doubles in): bucket counters, then indices
+  unsigned int* hist = reinterpret_cast<unsigned int*>(lds + sort_off);
+  uint16_t* idx = reinterpret_cast<uint16_t*>(hist + kBucketMax);
   const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
   int cur = -1;  // instance whose tables are in LDS (block-uniform)
   for (;;) {
@@ -94,11 +159,17 @@ __global__ void __launch_bounds__(1024)
       }
       const Geom g = insts[tl.inst].g;
       const int64_t n = tl.end - tl.begin;
+      const uint16_t* lord = nullptr;
+      if (DIM == 2 && sort_off > 0 && n <= kBucketTileMax) {  // (block-uniform)
+        tile_bucket_order(pos0, tl.begin, (int)n, g, hist, idx);
+        lord = idx;
+      }
       const int64_t per = (n + nw - 1) / nw;
       const int64_t b = tl.begin + (int64_t)w * per;
       const int64_t e = b + per < tl.end ? b + per : tl.end;
       if (b < e)
-        integrate_range<DIM, TESS, POW2, 0, true>(b, e, pos0, out, lds, lds + nC * NE, h, nT, nS, 0, g, nfb_out);
+        integrate_range<DIM, TESS, POW2, 0, true>(b, e, pos0, out, lds, lds + nC * NE, h, nT, nS, 0, g, nfb_out, 0,
+                                                  lord, tl.begin);
     }
   }
 }

@@ -432,13 +432,15 @@ __device__ __forceinline__ int64_t tile_order(int64_t i, int w, int64_t nTiled) 
 }
 
 // One wave integrates the points [begin, end) of the processing order
-// (indices into pos0, out and nfb_out; with gw > 0 through tile_order).
+// (indices into pos0, out and nfb_out; with gw > 0 through tile_order, with
+// lord through lbase + lord[i - lbase], a tile's order held in LDS).
 // Wave-uniform arguments; every lane of the wave calls it.
 template <int DIM, int TESS, bool POW2, int MODE, bool FAST = false>
 __device__ __forceinline__ void integrate_range(int64_t begin, int64_t end, const double* __restrict__ pos0,
                                                 double* __restrict__ out, const double* Ttab, const double* Atab,
                                                 double h, int nT, int nS, int64_t nPts, const Geom& g,
-                                                int32_t* __restrict__ nfb_out, int gw = 0) {
+                                                int32_t* __restrict__ nfb_out, int gw = 0,
+                                                const uint16_t* lord = nullptr, int64_t lbase = 0) {
   const int64_t nTiled = gw > 0 ? nPts / (8 * (int64_t)gw) * (8 * (int64_t)gw) : 0;
   constexpr int NE = DIM * (DIM + 1);
   int64_t next = begin;  // wave-uniform cursor into [begin, end)
@@ -458,7 +460,7 @@ __device__ __forceinline__ void integrate_range(int64_t begin, int64_t end, cons
       if (!active) {
         const int64_t cand = next + rank_in(idle);
         if (cand < end) {
-          my = gw > 0 ? tile_order(cand, gw, nTiled) : cand;
+          my = gw > 0 ? tile_order(cand, gw, nTiled) : (lord ? lbase + lord[cand - lbase] : cand);
           active = true;
           load_pt<DIM>(pos0, my, p);
           c = cell_of<DIM, TESS, POW2>(p, g);
