@@ -875,12 +875,12 @@ int batch_launch(cpab_batch* b, hipStream_t s, const cpab_batch_sched& q, const 
   // occupancy of a 16 x 16 x 4-triangle batch, ran its best schedule 4 %
   // slower, substeps being a quarter of its trips)
   size_t lds = (size_t)2 * b->maxNC * NE * sizeof(double);
-  // 2D tiles of up to kBucketTileMax points are taken in an order by position
-  // built in LDS after the tables (measured on MI355X, 1 024 instances of
-  // uniform random points, tiles of 16 384: 54.4 -> 43 ms)
+  // tiles of up to kBucketTileMax points are taken in an order by position
+  // built in LDS after the tables (measured on MI355X, 1 024 2D instances of
+  // uniform random points, tiles of 16 384: 54.4 -> 43.9 ms)
   int sort_off = 0;
   const size_t lds_sorted = lds + sizeof(unsigned int) * kBucketMax + sizeof(uint16_t) * (size_t)q.tile;
-  if (DIM == 2 && q.tile <= kBucketTileMax && lds_sorted <= (size_t)ctx->max_lds) {
+  if (q.tile <= kBucketTileMax && lds_sorted <= (size_t)ctx->max_lds) {
     sort_off = 2 * b->maxNC * NE;
     lds = lds_sorted;
   }
@@ -1076,10 +1076,21 @@ int cpab_batch_destroy(cpab_batch* b) {
 namespace {
 
 template <int DIM, int TESS, bool POW2>
-int items_launch(cpab_ctx* ctx, const WorkItem* items, unsigned int* queue, int nItems, int maxNC,
+int items_launch(cpab_ctx* ctx, const WorkItem* items, unsigned int* queue, int nItems, int maxNC, int64_t maxLen,
                  const cpab_batch_sched& q, double h, int nT, int nS) {
   constexpr int NE = DIM * (DIM + 1);
-  const size_t lds = (size_t)2 * maxNC * NE * sizeof(double);
+  size_t lds = (size_t)2 * maxNC * NE * sizeof(double);
+  // 1D items of up to kBucketTileMax points are taken in an order by position
+  // built in LDS after the tables (tile_bucket_order).  Measured on MI355X in
+  // the served mix: 1D requests of random points p50 0.49 -> 0.44 ms; the 2D
+  // and 3D requests there are grids whose items are whole rows, already in
+  // an order by position, and the sort's passes made them 1-3 % slower.
+  int sort_off = 0;
+  const size_t lds_sorted = lds + sizeof(unsigned int) * kBucketMax + sizeof(uint16_t) * (size_t)maxLen;
+  if (DIM == 1 && maxLen <= kBucketTileMax && lds_sorted <= (size_t)ctx->max_lds) {
+    sort_off = 2 * maxNC * NE;
+    lds = lds_sorted;
+  }
   auto k = k_integrate_items<DIM, TESS, POW2>;
   allow_lds(k, lds);
   int per_cu = 0;
@@ -1089,7 +1100,8 @@ int items_launch(cpab_ctx* ctx, const WorkItem* items, unsigned int* queue, int 
   const int64_t chunks = ((int64_t)nItems + q.chunk - 1) / q.chunk;
   if (grid > chunks) grid = (int)chunks;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, ctx->stream, queue);
-  hipLaunchKernelGGL(k, dim3(grid), dim3(q.block), lds, ctx->stream, items, nItems, q.chunk, queue, h, nT, nS);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(q.block), lds, ctx->stream, items, nItems, q.chunk, queue, h, nT, nS,
+                     sort_off);
   HIP_OK(ctx, hipGetLastError());
   return CPAB_OK;
 }
@@ -1139,7 +1151,8 @@ int item_convert(cpab_ctx* ctx, int dim, int tess, const cpab_work_item& q2, int
 // Argument checks shared by the item entries; the schedule in *q, the
 // largest tessellation and the increment kind of the list.
 int items_check(cpab_ctx* ctx, int dim, int tess, int nItems, const cpab_work_item* items, int nTimeSteps,
-                int nStepsODE, const cpab_batch_sched* sched, cpab_batch_sched* q, int* maxNC, bool* pow2_all) {
+                int nStepsODE, const cpab_batch_sched* sched, cpab_batch_sched* q, int* maxNC, bool* pow2_all,
+                int64_t* maxLen) {
   if (nItems < 0 || (nItems > 0 && !items)) return fail(ctx, CPAB_ERR_ARG, "bad item list");
   if (nTimeSteps < 0 || nStepsODE < 0) return fail(ctx, CPAB_ERR_ARG, "negative step counts");
   *q = sched ? *sched : cpab_batch_sched{16384, 1024, 1};
@@ -1147,6 +1160,7 @@ int items_check(cpab_ctx* ctx, int dim, int tess, int nItems, const cpab_work_it
     return fail(ctx, CPAB_ERR_ARG, "schedule: block a multiple of 64 up to 1024, chunk >= 1");
   bool all = true, any = false;
   int mx = 0;
+  int64_t len = 0;
   for (int i = 0; i < nItems; i++) {
     WorkItem W;
     bool pow2;
@@ -1155,6 +1169,7 @@ int items_check(cpab_ctx* ctx, int dim, int tess, int nItems, const cpab_work_it
     all = all && pow2;
     any = any || pow2;
     if (W.nC > mx) mx = W.nC;
+    if (W.end - W.begin > len) len = W.end - W.begin;
   }
   if (any && !all)
     return fail(ctx, CPAB_ERR_UNSUPPORTED, "items mix power-of-two and other increments: launch each kind apart");
@@ -1163,16 +1178,18 @@ int items_check(cpab_ctx* ctx, int dim, int tess, int nItems, const cpab_work_it
     return fail(ctx, CPAB_ERR_UNSUPPORTED, "the largest tessellation's tables exceed LDS");
   *maxNC = mx;
   *pow2_all = all;
+  *maxLen = len;
   return CPAB_OK;
 }
 
 int items_dispatch(cpab_ctx* ctx, const WorkItem* d_items, unsigned int* queue, int dim, int tess, bool pow2_all,
-                   int nItems, int maxNC, const cpab_batch_sched& q, double dt, int nTimeSteps, int nStepsODE) {
+                   int nItems, int maxNC, int64_t maxLen, const cpab_batch_sched& q, double dt, int nTimeSteps,
+                   int nStepsODE) {
   const double h = dtdiv(dt, nStepsODE);
   int r = CPAB_OK;
 #define ITEMS_CASE(D_, T_, P_)                                                                                    \
   if (dim == D_ && tess == T_ && pow2_all == P_)                                                                 \
-    r = items_launch<D_, T_, P_>(ctx, d_items, queue, nItems, maxNC, q, h, nTimeSteps, nStepsODE);               \
+    r = items_launch<D_, T_, P_>(ctx, d_items, queue, nItems, maxNC, maxLen, q, h, nTimeSteps, nStepsODE);       \
   else
   ITEMS_CASE(1, TESS_II, true)
   ITEMS_CASE(1, TESS_II, false)
@@ -1199,7 +1216,8 @@ int cpab_calc_T_items(cpab_ctx* ctx, int dim, int tess, int nItems, const cpab_w
   cpab_batch_sched q;
   int maxNC = 0;
   bool pow2_all = true;
-  int r = items_check(ctx, dim, tess, nItems, items, nTimeSteps, nStepsODE, sched, &q, &maxNC, &pow2_all);
+  int64_t maxLen = 0;
+  int r = items_check(ctx, dim, tess, nItems, items, nTimeSteps, nStepsODE, sched, &q, &maxNC, &pow2_all, &maxLen);
   if (r) return r;
   if (nItems == 0) return CPAB_OK;
   std::vector<WorkItem>& hv = ctx->h_items;
@@ -1224,7 +1242,8 @@ int cpab_calc_T_items(cpab_ctx* ctx, int dim, int tess, int nItems, const cpab_w
   HIP_OK(ctx, hipMemcpyAsync(ctx->items, hv.data(), sizeof(WorkItem) * nItems, hipMemcpyHostToDevice, ctx->stream));
   hipEvent_t a = nullptr, e = nullptr;
   if ((r = prof_begin(ctx, &a, &e))) return r;
-  r = items_dispatch(ctx, ctx->items, ctx->queue, dim, tess, pow2_all, nItems, maxNC, q, dt, nTimeSteps, nStepsODE);
+  r = items_dispatch(ctx, ctx->items, ctx->queue, dim, tess, pow2_all, nItems, maxNC, maxLen, q, dt, nTimeSteps,
+                     nStepsODE);
   if (r) return r;
   return prof_end(ctx, a, e);
 }
@@ -1241,7 +1260,8 @@ int cpab_calc_T_items_noalloc(cpab_ctx* ctx, int dim, int tess, int nItems, cons
   cpab_batch_sched q;
   int maxNC = 0;
   bool pow2_all = true;
-  int r = items_check(ctx, dim, tess, nItems, items, nTimeSteps, nStepsODE, sched, &q, &maxNC, &pow2_all);
+  int64_t maxLen = 0;
+  int r = items_check(ctx, dim, tess, nItems, items, nTimeSteps, nStepsODE, sched, &q, &maxNC, &pow2_all, &maxLen);
   if (r) return r;
   if (nItems == 0) return CPAB_OK;
   unsigned int* queue = reinterpret_cast<unsigned int*>(ctx->static_scratch);
@@ -1258,7 +1278,8 @@ int cpab_calc_T_items_noalloc(cpab_ctx* ctx, int dim, int tess, int nItems, cons
     }
     HIP_OK(ctx, hipMemcpyAsync(d_items + i0, buf, sizeof(WorkItem) * n, hipMemcpyHostToDevice, ctx->stream));
   }
-  return items_dispatch(ctx, d_items, queue, dim, tess, pow2_all, nItems, maxNC, q, dt, nTimeSteps, nStepsODE);
+  return items_dispatch(ctx, d_items, queue, dim, tess, pow2_all, nItems, maxNC, maxLen, q, dt, nTimeSteps,
+                        nStepsODE);
 }
 
 // ------------------------------------------------------------- MCMC

@@ -64,9 +64,9 @@ __global__ void k_batch_reset(unsigned int* queue) {
   if (threadIdx.x == 0) *queue = 0u;
 }
 
-// Processing order of a 2D tile's points by position: a counting sort of
-// the tile's point indices over an nb x nb grid of buckets on the
-// instance's domain (nb a power of two up to 32, at least 16 points a
+// Processing order of a tile's points by position: a counting sort of the
+// tile's point indices over a grid of buckets on the domain (nb per axis, a
+// power of two up to 1 024 in 1D, 32 in 2D, 8 in 3D, at least 16 points a
 // bucket on average), buckets in row order, built in LDS by the whole block.
 // Neighbouring points flow alike and cross facets at about the same steps,
 // so a wave's lanes then need closer numbers of trips than for points in
@@ -74,28 +74,37 @@ __global__ void k_batch_reset(unsigned int* queue) {
 // differ from run to run: that changes which lane integrates a point and
 // when, never the point's result.
 constexpr int kBucketTileMax = 16384;  // the most points of a sorted tile (16-bit indices)
-constexpr int kBucketMax = 1024;       // buckets (32 x 32)
+constexpr int kBucketMax = 1024;       // buckets
 
-__device__ __forceinline__ int bucket_of(const double* p, double sx, double sy, int nb) {
+template <int DIM>
+__device__ __forceinline__ int bucket_of(const double* p, const double* sc, int nb) {
   // (NaN coordinates go to a border bucket: fmin/fmax return the other operand)
-  const double fx = fmax(0.0, fmin(p[0] * sx, (double)(nb - 1)));
-  const double fy = fmax(0.0, fmin(p[1] * sy, (double)(nb - 1)));
-  return (int)fy * nb + (int)fx;
+  int b = 0;
+#pragma unroll
+  for (int d = DIM - 1; d >= 0; d--) b = b * nb + (int)fmax(0.0, fmin(p[d] * sc[d], (double)(nb - 1)));
+  return b;
 }
 
+template <int DIM>
 __device__ void tile_bucket_order(const double* __restrict__ pos0, int64_t begin, int n, const Geom& g,
                                   unsigned int* hist, uint16_t* idx) {
-  int nb = 1;
-  while (nb < 32 && 16 * (4 * nb * nb) <= n) nb *= 2;
-  const int nbk = nb * nb;
-  const double sx = (double)nb / ((double)g.nC0 * g.inc_x), sy = (double)nb / ((double)g.nC1 * g.inc_y);
+  constexpr int NBMAX = DIM == 1 ? 1024 : (DIM == 2 ? 32 : 8);
+  int nb = 1, nbk = 1;
+  while (nb < NBMAX && 16 * (nbk << DIM) <= n) {
+    nb <<= 1;
+    nbk <<= DIM;
+  }
+  double sc[DIM];
+  sc[0] = (double)nb / ((double)g.nC0 * g.inc_x);
+  if (DIM > 1) sc[DIM > 1 ? 1 : 0] = (double)nb / ((double)g.nC1 * g.inc_y);
+  if (DIM > 2) sc[DIM > 2 ? 2 : 0] = (double)nb / ((double)g.nC2 * g.inc_z);
   __syncthreads();  // every wave is done with the previous tile's order
   for (int i = threadIdx.x; i < nbk; i += blockDim.x) hist[i] = 0u;
   __syncthreads();
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    double p[2];
-    load_pt<2>(pos0, begin + j, p);
-    atomicAdd(&hist[bucket_of(p, sx, sy, nb)], 1u);
+    double p[DIM];
+    load_pt<DIM>(pos0, begin + j, p);
+    atomicAdd(&hist[bucket_of<DIM>(p, sc, nb)], 1u);
   }
   __syncthreads();
   if (threadIdx.x < 64) {  // exclusive scan by the first wave, each lane a run of buckets
@@ -118,9 +127,9 @@ __device__ void tile_bucket_order(const double* __restrict__ pos0, int64_t begin
   }
   __syncthreads();
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    double p[2];
-    load_pt<2>(pos0, begin + j, p);
-    idx[atomicAdd(&hist[bucket_of(p, sx, sy, nb)], 1u)] = (uint16_t)j;
+    double p[DIM];
+    load_pt<DIM>(pos0, begin + j, p);
+    idx[atomicAdd(&hist[bucket_of<DIM>(p, sc, nb)], 1u)] = (uint16_t)j;
   }
   __syncthreads();
 }
@@ -134,8 +143,8 @@ __global__ void __launch_bounds__(1024)
   constexpr int NE = DIM * (DIM + 1);
   extern __shared__ double lds[];
   __shared__ int s_t0;
-  // sort_off > 0 (2D): the tiles' processing order by position, in LDS
-  // after the tables (sort_off doubles in): bucket counters, then indices
+  // sort_off > 0: the tiles' processing order by position, in LDS after
+  // the tables (sort_off doubles in): bucket counters, then indices
   unsigned int* hist = reinterpret_cast<unsigned int*>(lds + sort_off);
   uint16_t* idx = reinterpret_cast<uint16_t*>(hist + kBucketMax);
   const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
@@ -160,8 +169,8 @@ __global__ void __launch_bounds__(1024)
       const Geom g = insts[tl.inst].g;
       const int64_t n = tl.end - tl.begin;
       const uint16_t* lord = nullptr;
-      if (DIM == 2 && sort_off > 0 && n <= kBucketTileMax) {  // (block-uniform)
-        tile_bucket_order(pos0, tl.begin, (int)n, g, hist, idx);
+      if (sort_off > 0 && n <= kBucketTileMax) {  // (block-uniform)
+        tile_bucket_order<DIM>(pos0, tl.begin, (int)n, g, hist, idx);
         lord = idx;
       }
       const int64_t per = (n + nw - 1) / nw;
@@ -194,10 +203,13 @@ struct WorkItem {
 template <int DIM, int TESS, bool POW2>
 __global__ void __launch_bounds__(1024)
     k_integrate_items(const WorkItem* __restrict__ items, int nItems, int chunk, unsigned int* __restrict__ queue,
-                      double h, int nT, int nS) {
+                      double h, int nT, int nS, int sort_off) {
   constexpr int NE = DIM * (DIM + 1);
   extern __shared__ double lds[];
   __shared__ int s_t0;
+  // sort_off > 0: each item's points in an order by position (tile_bucket_order)
+  unsigned int* hist = reinterpret_cast<unsigned int*>(lds + sort_off);
+  uint16_t* idx = reinterpret_cast<uint16_t*>(hist + kBucketMax);
   const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
   const double* cur = nullptr;  // Trels in LDS (block-uniform)
   for (;;) {
@@ -223,11 +235,17 @@ __global__ void __launch_bounds__(1024)
       }
       const Geom g = it.g;
       const int64_t n = it.end - it.begin;
+      const uint16_t* lord = nullptr;
+      if (sort_off > 0 && n <= kBucketTileMax) {  // (block-uniform)
+        tile_bucket_order<DIM>(it.pos0, it.begin, (int)n, g, hist, idx);
+        lord = idx;
+      }
       const int64_t per = (n + nw - 1) / nw;
       const int64_t b = it.begin + (int64_t)w * per;
       const int64_t e = b + per < it.end ? b + per : it.end;
       if (b < e)
-        integrate_range<DIM, TESS, POW2, 0, false>(b, e, it.pos0, it.out, lds, lds + nC * NE, h, nT, nS, 0, g, nullptr);
+        integrate_range<DIM, TESS, POW2, 0, false>(b, e, it.pos0, it.out, lds, lds + nC * NE, h, nT, nS, 0, g, nullptr,
+                                                   0, lord, it.begin);
     }
   }
 }
