@@ -131,19 +131,20 @@ __device__ __forceinline__ int cell2(const double* p, const Geom& g) {
   }
   const double px = p[0], py = p[1];
   if (POW2 && FAST) {
+    // The interior form's cell, formed before the test that allows it so
+    // that the test's scalar chain and branch are off the cell's critical
+    // path (a lane outside gets a value that is not used).
+    const double sx = px * g.rinc_x, sy = py * g.rinc_y;  // > 0 inside: fract as in axis_simplex
+    const double x = __builtin_amdgcn_fract(sx), y = __builtin_amdgcn_fract(sy);
+    const int c = ((int)sx + mul24((int)sy, g.nC0)) * 4;
+    const double omx = 1.0 - x;
+    const int cf = c + ((x < y) ? ((omx < y) ? 2 : 3) : ((omx < y) ? 1 : 0));
     // the strict-interior test as lane masks (fcmp predicates: 2 ordered
     // greater, 4 ordered less; inactive lanes read 0), combined in scalar code
     const uint64_t ok = __builtin_amdgcn_fcmp(px, 0.0, 2) & __builtin_amdgcn_fcmp(px, g.top_x, 4) &
                         __builtin_amdgcn_fcmp(py, 0.0, 2) & __builtin_amdgcn_fcmp(py, g.top_y, 4);
     const uint64_t ex = __builtin_amdgcn_read_exec();
-    if ((ok & ex) == ex) {
-      const double sx = px * g.rinc_x, sy = py * g.rinc_y;  // > 0: fract as in axis_simplex
-      const double x = __builtin_amdgcn_fract(sx), y = __builtin_amdgcn_fract(sy);
-      const int c = ((int)sx + mul24((int)sy, g.nC0)) * 4;
-      const double omx = 1.0 - x;
-      if (x < y) return c + ((omx < y) ? 2 : 3);
-      return c + ((omx < y) ? 1 : 0);
-    }
+    if ((ok & ex) == ex) return cf;
   }
   const double p0 = dmin(g.top_x, dmax(0.0, px));
   const double p1 = dmin(g.top_y, dmax(0.0, py));
