@@ -123,7 +123,7 @@ __device__ __forceinline__ int cell1(const double* p, const Geom& g) {
 // speculation +1-2 %, the 512x512 calc_T +3 %, the 1 024-instance batch
 // +1.5 %; but a 65536-point calc_T -28 % and the serving mix's work items
 // -5 %.  So each launch chooses it.
-template <int TESS, bool POW2, bool FAST = false>
+template <int TESS, bool POW2, bool FAST = false, bool EARLY = true>
 __device__ __forceinline__ int cell2(const double* p, const Geom& g) {
   if (TESS == TESS_II) {
     return axis_box<POW2>(p[0], g.nC0, g.inc_x, g.rinc_x) +
@@ -131,20 +131,36 @@ __device__ __forceinline__ int cell2(const double* p, const Geom& g) {
   }
   const double px = p[0], py = p[1];
   if (POW2 && FAST) {
-    // The interior form's cell, formed before the test that allows it so
-    // that the test's scalar chain and branch are off the cell's critical
-    // path (a lane outside gets a value that is not used).
-    const double sx = px * g.rinc_x, sy = py * g.rinc_y;  // > 0 inside: fract as in axis_simplex
-    const double x = __builtin_amdgcn_fract(sx), y = __builtin_amdgcn_fract(sy);
-    const int c = ((int)sx + mul24((int)sy, g.nC0)) * 4;
-    const double omx = 1.0 - x;
-    const int cf = c + ((x < y) ? ((omx < y) ? 2 : 3) : ((omx < y) ? 1 : 0));
-    // the strict-interior test as lane masks (fcmp predicates: 2 ordered
-    // greater, 4 ordered less; inactive lanes read 0), combined in scalar code
-    const uint64_t ok = __builtin_amdgcn_fcmp(px, 0.0, 2) & __builtin_amdgcn_fcmp(px, g.top_x, 4) &
-                        __builtin_amdgcn_fcmp(py, 0.0, 2) & __builtin_amdgcn_fcmp(py, g.top_y, 4);
+    // The interior form's cell.  EARLY: formed before the test that allows
+    // it, so that the test's scalar chain and branch are off the cell's
+    // critical path (a lane outside gets a value that is not used; measured
+    // on MI355X: the 512 x 512 calc_T -1.5 %, one registration chain +1.2 %,
+    // the batch -5.5 %, but 256 registration chains one step per launch 2 %
+    // slower, so those launches form it after the test).
     const uint64_t ex = __builtin_amdgcn_read_exec();
-    if ((ok & ex) == ex) return cf;
+    if (EARLY) {
+      const double sx = px * g.rinc_x, sy = py * g.rinc_y;  // > 0 inside: fract as in axis_simplex
+      const double x = __builtin_amdgcn_fract(sx), y = __builtin_amdgcn_fract(sy);
+      const int c = ((int)sx + mul24((int)sy, g.nC0)) * 4;
+      const double omx = 1.0 - x;
+      const int cf = c + ((x < y) ? ((omx < y) ? 2 : 3) : ((omx < y) ? 1 : 0));
+      // the strict-interior test as lane masks (fcmp predicates: 2 ordered
+      // greater, 4 ordered less; inactive lanes read 0), combined in scalar code
+      const uint64_t ok = __builtin_amdgcn_fcmp(px, 0.0, 2) & __builtin_amdgcn_fcmp(px, g.top_x, 4) &
+                          __builtin_amdgcn_fcmp(py, 0.0, 2) & __builtin_amdgcn_fcmp(py, g.top_y, 4);
+      if ((ok & ex) == ex) return cf;
+    } else {
+      const uint64_t ok = __builtin_amdgcn_fcmp(px, 0.0, 2) & __builtin_amdgcn_fcmp(px, g.top_x, 4) &
+                          __builtin_amdgcn_fcmp(py, 0.0, 2) & __builtin_amdgcn_fcmp(py, g.top_y, 4);
+      if ((ok & ex) == ex) {
+        const double sx = px * g.rinc_x, sy = py * g.rinc_y;
+        const double x = __builtin_amdgcn_fract(sx), y = __builtin_amdgcn_fract(sy);
+        const int c = ((int)sx + mul24((int)sy, g.nC0)) * 4;
+        const double omx = 1.0 - x;
+        if (x < y) return c + ((omx < y) ? 2 : 3);
+        return c + ((omx < y) ? 1 : 0);
+      }
+    }
   }
   const double p0 = dmin(g.top_x, dmax(0.0, px));
   const double p1 = dmin(g.top_y, dmax(0.0, py));
@@ -230,10 +246,10 @@ __device__ __forceinline__ int cell3(const double* p, const Geom& g) {
   return c;
 }
 
-template <int DIM, int TESS, bool POW2, bool FAST = false>
+template <int DIM, int TESS, bool POW2, bool FAST = false, bool EARLY = true>
 __device__ __forceinline__ int cell_of(const double* p, const Geom& g) {
   if (DIM == 1) return cell1<POW2>(p, g);
-  if (DIM == 2) return cell2<TESS, POW2, FAST>(p, g);
+  if (DIM == 2) return cell2<TESS, POW2, FAST, EARLY>(p, g);
   return cell3<TESS, POW2>(p, g);
 }
 

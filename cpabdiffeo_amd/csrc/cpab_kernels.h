@@ -290,7 +290,7 @@ __device__ __forceinline__ void stage_tables_prep(double* lds, const PrepSrc& ps
 // and selecting, or chaining several Trel proposals speculatively so their
 // lookups overlap, are both slower than this.)
 // Returns true when the step is complete (the caller advances t).
-template <int DIM, int TESS, bool POW2, bool FAST = false>
+template <int DIM, int TESS, bool POW2, bool FAST = false, bool EARLY = true>
 __device__ __forceinline__ bool trip(double* p, int& c, int& k, const double*& row, int& nfb, const double* Ttab,
                                      const double* Atab, double h, int nS, const Geom& g) {
   constexpr int NE = DIM * (DIM + 1);
@@ -311,7 +311,7 @@ __device__ __forceinline__ bool trip(double* p, int& c, int& k, const double*& r
 #pragma unroll
     for (int d = 0; d < DIM; d++) y[d] = p[d] + y[d] * h;
   }
-  const int cy = cell_of<DIM, TESS, POW2, FAST>(y, g);
+  const int cy = cell_of<DIM, TESS, POW2, FAST, EARLY>(y, g);
   // (Measured on gfx950: the state update below written as selects instead
   // of branches ran 3-4 % slower here and 11 % slower for 256 registration
   // chains.)

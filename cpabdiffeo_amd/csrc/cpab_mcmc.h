@@ -687,7 +687,7 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
               pix = (int64_t)D.pixmap[my];
               p[0] = (double)(pix % D.W);
               p[1] = (double)(pix / D.W);
-              c = cell2<TESS, POW2, FAST>(p, g);
+              c = cell2<TESS, POW2, FAST, !SWAP>(p, g);
               row = Ttab + mul24(c, 6);
               t = 0;
               k = 0;
@@ -723,9 +723,9 @@ __global__ void __launch_bounds__(BLOCK) k_mcmc_step(McmcDesc D, int init) {
           continue;
         }
         for (;;) {  // two trips per iteration, as in integrate_range
-          if (trip<2, TESS, POW2, FAST>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) t++;
+          if (trip<2, TESS, POW2, FAST, !SWAP>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) t++;
           if (ballot(t >= tb) != 0) break;
-          if (trip<2, TESS, POW2, FAST>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) t++;
+          if (trip<2, TESS, POW2, FAST, !SWAP>(p, c, k, row, nfb, Ttab, Atab, h, nS, g)) t++;
           if (ballot(t >= tb) != 0) break;
         }
       }
