@@ -4,7 +4,7 @@ Bar: cell ids, velocities, integrated points, trajectories, remaps are
 bit-identical to the oracle when both start from the same per-cell tables
 (same IEEE operations in the same order).  The device matrix exponential
 uses the GPU's libm (exp/cosh/sinh/cos/sin), so end-to-end results that go
-through it are compared at 1e-10 except for points
+through it are compared at 1e-10 (the floating-point parity tolerance) except for points
 whose trajectory passed within 1e-12 of a facet, which are counted.
 """
 import os
